@@ -1,0 +1,77 @@
+// Device-side data structures and kernels for the Yee step on gfx950.
+//
+// Layout in HBM (one allocation each):
+//   E[2], H[2]  3·N f32, component-major, x fastest (ping-pong pair)
+//   material    N u8 (index into a ≤256-entry coefficient table)
+//   axis table  4 sections × (nx+ny+nz) f32:
+//               [primary gains x|y|z][dual gains x|y|z]
+//               [integer absorber x|y|z][half absorber x|y|z]
+//   intensity   N f64
+//
+// Arithmetic contract: every cell update is the reference's expression in
+// the reference's order (src/cpu.rs:319-331, 418-430), compiled with
+// -ffp-contract=off and IEEE division, so results are bit-identical to the
+// CPU sweep.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace dpa {
+
+// One injection as the kernels consume it: a box, a component, a value
+// already multiplied by the direction of time, and an optional offset into a
+// host-computed apodization-weight table laid out over the box, x fastest.
+struct alignas(16) InjDesc {
+    uint32_t origin[3];
+    uint32_t extent[3];
+    uint32_t component;
+    int32_t weight_offset;  // < 0: weight 1
+    float value;
+    uint32_t pad[3];
+};
+
+// Sections of the per-axis table.
+enum : int { kGainH = 0, kGainE = 1, kLossInt = 2, kLossHalf = 3 };
+
+struct StepArgs {
+    const float *e_in;
+    const float *h_in;
+    float *e_out;
+    float *h_out;
+    const uint8_t *material;
+    const float4 *coefficients;  // (eg, el, mg, ml) per material
+    const float *axis_table;     // see above
+    const float *weights;        // apodization weights
+    const InjDesc *injections;   // magnetic ones first, then electric ones
+    double *intensity;           // null when not accumulating
+    uint32_t n_magnetic;
+    uint32_t n_electric;
+    uint32_t material_count;
+    uint32_t nx, ny, nz;
+    uint64_t cells;
+    // fused-kernel tiling
+    uint32_t tiles_x, tiles_y, tiles_z, depth;
+    uint32_t tiles_total;
+    uint32_t xcd_remap;
+};
+
+// Tile geometry of the fused kernel: one wave of 64 lanes covers 62 owned
+// columns plus one halo column on each side.
+constexpr int kLanes = 64;
+constexpr int kOwnedX = kLanes - 2;
+
+void launch_fused(const StepArgs &args, int rows, hipStream_t stream);
+void launch_update_h(const StepArgs &args, int direction, hipStream_t stream);
+void launch_update_e(const StepArgs &args, int direction, hipStream_t stream);
+void launch_inject(float *field, uint64_t cells, uint32_t nx, uint32_t ny, const InjDesc &desc,
+                   const InjDesc *device_desc, const float *weights, hipStream_t stream);
+void launch_intensity(const float *e, double *intensity, uint64_t cells, hipStream_t stream);
+// f64 energy partials, one pair per block; returns the number of blocks.
+uint32_t launch_energy(const float *e, const float *h, const uint8_t *material,
+                       const float2 *eps_mu, const float *widths, uint32_t nx, uint32_t ny,
+                       uint32_t nz, double *partials, hipStream_t stream);
+uint32_t energy_blocks(uint32_t nx, uint32_t ny, uint32_t nz);
+
+}  // namespace dpa

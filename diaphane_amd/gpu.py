@@ -1,0 +1,212 @@
+"""The device solver: gpu::Simulation's counterpart (src/gpu.rs:159-799).
+
+A thin binding over the C ABI; every call goes to libdiaphane_amd.so, which
+runs the hand-written gfx950 kernels. There is no CPU fallback: without a
+device, ``headless_context`` raises.
+"""
+from __future__ import annotations
+
+import ctypes
+from ctypes import POINTER, byref, c_double, c_float, c_uint32, c_void_p
+from dataclasses import dataclass
+from typing import Optional
+
+import numpy as np
+
+from ._lib import DiaphaneError, check, lib
+from .scene import Axis, Scene
+from .timeline import Snapshot
+
+KERNEL_FUSED = 0
+KERNEL_TWO_PASS = 1
+
+BUFFER_ELECTRIC = 0
+BUFFER_MAGNETIC = 1
+BUFFER_INTENSITY = 2
+BUFFER_MATERIAL_INDEX = 3
+
+
+@dataclass
+class Energy:
+    """cpu::Energy (src/cpu.rs:52-62)."""
+
+    electric: float
+    magnetic: float
+
+    def total(self) -> float:
+        return self.electric + self.magnetic
+
+
+class Context:
+    """An opened HIP device (headless_context, src/gpu.rs:805-817)."""
+
+    def __init__(self, device: int = 0):
+        handle = c_void_p()
+        check(lib().dp_engine_create(device, byref(handle)))
+        self._handle = handle
+
+    @property
+    def handle(self) -> c_void_p:
+        return self._handle
+
+    def arch(self) -> str:
+        buf = ctypes.create_string_buffer(64)
+        check(lib().dp_engine_arch(self._handle, buf, 64))
+        return buf.value.decode()
+
+    def __del__(self):
+        handle = getattr(self, "_handle", None)
+        if handle and lib is not None:
+            try:
+                lib().dp_engine_destroy(handle)
+            except Exception:
+                pass
+            self._handle = None
+
+
+_default_context: Optional[Context] = None
+
+
+def headless_context(device: int = 0) -> Context:
+    """Opens (once) and returns the context for ``device``."""
+    global _default_context
+    if _default_context is None:
+        _default_context = Context(device)
+    return _default_context
+
+
+def _fptr(array: np.ndarray):
+    return array.ctypes.data_as(POINTER(c_float))
+
+
+class Simulation:
+    """A running simulation on the device."""
+
+    TIMEOUT_MS = 120_000
+
+    def __init__(self, context: Context, scene: Scene):
+        self._context = context
+        self._scene_c = scene.to_c()
+        handle = c_void_p()
+        check(lib().dp_sim_create(context.handle, byref(self._scene_c), byref(handle)))
+        self._handle = handle
+        self.grid = scene.grid
+        extent = (c_uint32 * 3)()
+        check(lib().dp_sim_extent(self._handle, extent))
+        self.extent = (int(extent[0]), int(extent[1]), int(extent[2]))
+        self.cells = self.extent[0] * self.extent[1] * self.extent[2]
+
+    def close(self) -> None:
+        handle = getattr(self, "_handle", None)
+        if handle:
+            lib().dp_sim_destroy(handle)
+            self._handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # -- reference surface ------------------------------------------------
+    def step_count(self) -> int:
+        return int(lib().dp_sim_step_count(self._handle))
+
+    def time(self) -> float:
+        return float(lib().dp_sim_time(self._handle))
+
+    def time_step(self) -> float:
+        return float(lib().dp_sim_time_step(self._handle))
+
+    def field_bytes(self) -> int:
+        return int(lib().dp_sim_field_bytes(self._handle))
+
+    def bytes_per_step(self) -> int:
+        return int(lib().dp_sim_bytes_per_step(self._handle))
+
+    def is_reversible(self) -> bool:
+        return bool(lib().dp_sim_is_reversible(self._handle))
+
+    def advance_by(self, steps: int) -> None:
+        check(lib().dp_sim_advance(self._handle, int(steps)))
+
+    def reverse_by(self, steps: int) -> None:
+        check(lib().dp_sim_reverse(self._handle, int(steps)))
+
+    def wait(self, timeout_ms: int = 0) -> None:
+        check(lib().dp_sim_wait(self._handle, int(timeout_ms)))
+
+    def reset(self) -> None:
+        check(lib().dp_sim_reset(self._handle))
+
+    def set_accumulate_intensity(self, on: bool) -> None:
+        check(lib().dp_sim_set_accumulate_intensity(self._handle, 1 if on else 0))
+
+    def accumulated_steps(self) -> int:
+        return int(lib().dp_sim_accumulated_steps(self._handle))
+
+    def read_electric(self) -> np.ndarray:
+        out = np.empty(3 * self.cells, dtype=np.float32)
+        check(lib().dp_sim_read_electric(self._handle, _fptr(out), out.size))
+        return out
+
+    def read_magnetic(self) -> np.ndarray:
+        out = np.empty(3 * self.cells, dtype=np.float32)
+        check(lib().dp_sim_read_magnetic(self._handle, _fptr(out), out.size))
+        return out
+
+    def read_intensity(self) -> np.ndarray:
+        out = np.empty(self.cells, dtype=np.float32)
+        check(lib().dp_sim_read_intensity(self._handle, _fptr(out), out.size))
+        return out
+
+    def read_intensity_f64(self) -> np.ndarray:
+        out = np.empty(self.cells, dtype=np.float64)
+        check(lib().dp_sim_read_intensity_f64(self._handle, out.ctypes.data_as(POINTER(c_double)), out.size))
+        return out
+
+    def component(self, values: np.ndarray, axis: Axis) -> np.ndarray:
+        a = int(axis)
+        return values[a * self.cells:(a + 1) * self.cells]
+
+    def write_fields(self, electric: np.ndarray, magnetic: np.ndarray, step: int) -> None:
+        e = np.ascontiguousarray(electric, dtype=np.float32)
+        h = np.ascontiguousarray(magnetic, dtype=np.float32)
+        if e.size != h.size:
+            raise DiaphaneError(-4, "snapshot was taken from a different grid")
+        check(lib().dp_sim_write_fields(self._handle, _fptr(e), _fptr(h), e.size, int(step)))
+
+    def energy(self) -> Energy:
+        ue, uh = c_double(), c_double()
+        check(lib().dp_sim_energy(self._handle, byref(ue), byref(uh)))
+        return Energy(ue.value, uh.value)
+
+    def device_ptr(self, which: int) -> int:
+        out = c_void_p()
+        check(lib().dp_sim_device_ptr(self._handle, which, byref(out)))
+        return int(out.value or 0)
+
+    # -- Steppable (src/timeline.rs:63-72) ---------------------------------
+    def snapshot(self) -> Snapshot:
+        return Snapshot(self.step_count(), self.read_electric(), self.read_magnetic())
+
+    def restore(self, snapshot: Snapshot) -> None:
+        self.write_fields(snapshot.electric, snapshot.magnetic, snapshot.step)
+
+    # -- engine controls ---------------------------------------------------
+    def set_kernel(self, kernel: int) -> None:
+        check(lib().dp_sim_set_kernel(self._handle, kernel))
+
+    def set_tile(self, rows: int) -> None:
+        check(lib().dp_sim_set_tile(self._handle, rows))
+
+    def record_event(self, slot: int) -> None:
+        check(lib().dp_sim_record_event(self._handle, slot))
+
+    def elapsed_ms(self, start: int, end: int) -> float:
+        ms = c_float()
+        check(lib().dp_sim_elapsed_ms(self._handle, start, end, byref(ms)))
+        return float(ms.value)
+
+    def launch_count(self) -> int:
+        return int(lib().dp_sim_launch_count(self._handle))

@@ -1,0 +1,278 @@
+/*
+ * diaphane_amd — MI355X (gfx950) engine for diaphane's 3D Yee FDTD solver.
+ *
+ * C ABI. Plain structs, pointers and sizes; no C++ or torch types cross it.
+ * Every entry point that can fail returns an int status (DP_OK == 0) and
+ * leaves a thread-local message in dp_last_error(). The messages keep the
+ * wording of the reference's panics so a binding can raise the same text.
+ *
+ * Citations are to kvark/diaphane (paths relative to the crate root).
+ *
+ * Ownership: the caller owns every host array it passes in (copied at the
+ * call) or out (filled at the call). The library owns device memory. A dp_sim
+ * is externally synchronized; separate sims may be driven from separate
+ * threads.
+ */
+#ifndef DIAPHANE_AMD_H
+#define DIAPHANE_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------------ */
+/* Status codes                                                             */
+/* ------------------------------------------------------------------------ */
+enum {
+    DP_OK = 0,
+    DP_ERR_INVALID = -1,     /* bad argument / invalid grid or scene        */
+    DP_ERR_LOSSY = -2,       /* reversal of a lossy scene (cpu.rs:253-260)  */
+    DP_ERR_PAST_START = -3,  /* reversing past step 0 (gpu.rs:509)          */
+    DP_ERR_SIZE = -4,        /* buffer length mismatch (gpu.rs:665-669)     */
+    DP_ERR_TIMEOUT = -5,     /* wait timed out (gpu.rs:616-622)             */
+    DP_ERR_DEVICE = -6,      /* HIP runtime failure                         */
+    DP_ERR_UNSUPPORTED = -7, /* outside what this build supports            */
+    DP_ERR_BUDGET = -8       /* device-memory budget exhausted (serving)    */
+};
+
+/* ------------------------------------------------------------------------ */
+/* Scene description (the input format). Mirrors the reference's Scene,    */
+/* Grid/GridSpec, Material, Shape, Source and Waveform field for field.     */
+/* Positions are metres from the domain centre, as in the reference.        */
+/* ------------------------------------------------------------------------ */
+enum { DP_AXIS_X = 0, DP_AXIS_Y = 1, DP_AXIS_Z = 2 };
+
+/* Refinement, src/grid.rs:313-336. size[a] <= 0 opts axis a out. */
+typedef struct dp_refinement {
+    float center[3];
+    float size[3];
+    float cell_size;
+} dp_refinement;
+
+/* GridSpec, src/grid.rs:381-392: the recipe, resolved by grade(). */
+typedef struct dp_grid_spec {
+    uint32_t extent[3]; /* cells the domain has at cell_size            */
+    float cell_size;    /* coarse cell size, metres                     */
+    float courant;      /* S = c·Δt/Δ_ref, (0, 0.57735026]              */
+    float max_ratio;    /* growth cap between neighbours (default 1.15) */
+    const dp_refinement *refinements;
+    uint32_t refinement_count;
+} dp_grid_spec;
+
+/* Material, src/material.rs:26-39. conductivity = +inf is a PEC. */
+typedef struct dp_material {
+    float relative_permittivity;
+    float relative_permeability;
+    float conductivity;
+    float magnetic_loss;
+} dp_material;
+
+/* Shape, src/scene.rs:32-56. Later shapes overwrite earlier ones. */
+enum { DP_SHAPE_BLOCK = 0, DP_SHAPE_SPHERE = 1, DP_SHAPE_SLAB = 2 };
+typedef struct dp_shape {
+    uint32_t kind;
+    uint32_t material;
+    float center[3];  /* block, sphere  */
+    float size[3];    /* block          */
+    float radius;     /* sphere         */
+    uint32_t axis;    /* slab           */
+    float offset;     /* slab           */
+    float thickness;  /* slab           */
+} dp_shape;
+
+/* Waveform, src/source.rs:31-52, plus a differentiated Gaussian.
+ *   RICKER:          frequency = peak_frequency, delay
+ *   GAUSSIAN_PULSE:  frequency = center_frequency, cycles = duration_cycles
+ *                    (FWHM in carrier periods), delay
+ *   CONTINUOUS_WAVE: frequency, cycles = ramp_cycles
+ *   DIFF_GAUSSIAN:   width = σ in seconds, delay; value
+ *                    −u·exp(½(1 − u²)) with u = (t − delay)/σ (peak |1| at
+ *                    u = ∓1); dominant frequency 1/(2πσ). No counterpart in
+ *                    the reference; both the oracle and the engine
+ *                    evaluate it on the host like every other waveform. */
+enum {
+    DP_WAVE_RICKER = 0,
+    DP_WAVE_GAUSSIAN_PULSE = 1,
+    DP_WAVE_CONTINUOUS_WAVE = 2,
+    DP_WAVE_DIFF_GAUSSIAN = 3
+};
+typedef struct dp_waveform {
+    uint32_t kind;
+    float frequency;
+    float cycles;
+    float delay;
+    float width;
+} dp_waveform;
+
+/* Source + SourceShape, src/source.rs:136-197. */
+enum { DP_SOURCE_POINT = 0, DP_SOURCE_SHEET = 1, DP_SOURCE_PLANE_WAVE = 2 };
+typedef struct dp_source {
+    uint32_t shape;
+    uint32_t polarization; /* driven E component                     */
+    float at[3];           /* point                                  */
+    float velocity[3];     /* point, m/s                             */
+    uint32_t axis;         /* sheet normal / plane-wave travel axis  */
+    float offset;          /* sheet, plane wave                      */
+    float waist;           /* sheet                                  */
+    dp_waveform waveform;
+    float amplitude;
+} dp_source;
+
+/* Boundary, src/boundary.rs:53-75. */
+enum { DP_BOUNDARY_PEC = 0, DP_BOUNDARY_ABSORBING = 1 };
+
+/* Scene, src/scene.rs:116-122. materials[0] must be vacuum. */
+typedef struct dp_scene {
+    dp_grid_spec grid;
+    uint32_t boundary;
+    uint32_t absorber_thickness;   /* cells, absorbing only       */
+    float target_reflection;       /* absorbing only, in (0, 1)   */
+    const dp_material *materials;
+    uint32_t material_count;
+    const dp_shape *shapes;
+    uint32_t shape_count;
+    const dp_source *sources;
+    uint32_t source_count;
+} dp_scene;
+
+/* ------------------------------------------------------------------------ */
+/* Host-side scene resolution (no device needed).                           */
+/* ------------------------------------------------------------------------ */
+
+/* Message of the last failed call on this thread ("" if none). */
+const char *dp_last_error(void);
+
+/* Library version string. */
+const char *dp_version(void);
+
+/* Resolves a GridSpec: Grid::build (src/grid.rs:466-500, grade() at
+ * :791-888). extent_out gets the resolved cell counts; widths_out (may be
+ * NULL) gets the primary widths of x, then y, then z
+ * (extent[0]+extent[1]+extent[2] floats, widths_capacity checked);
+ * time_step_out (may be NULL) gets Grid::time_step (src/grid.rs:597-599). */
+int dp_grid_resolve(const dp_grid_spec *grid, uint32_t extent_out[3],
+                    float *widths_out, size_t widths_capacity,
+                    float *time_step_out);
+
+/* Scene::validate (src/scene.rs:254-369) for the recoverable problems;
+ * DP_OK or DP_ERR_INVALID with the reason in dp_last_error(). */
+int dp_scene_validate(const dp_scene *scene);
+
+/* Scene::material_indices (src/scene.rs:181-206), widened to u32 so it can
+ * be compared with the reference's Vec<u32>. count = resolved cell count. */
+int dp_scene_material_indices(const dp_scene *scene, uint32_t *out,
+                              size_t count);
+
+/* One injection as Source::injections resolves it (src/source.rs:409-439),
+ * already sign-scaled. weight_kind 0: every cell weight 1; 1: Gaussian
+ * apodization (center, inverse_waist_squared) evaluated on the host. */
+typedef struct dp_injection {
+    uint32_t origin[3];
+    uint32_t extent[3];
+    uint32_t component;
+    uint32_t magnetic;
+    float value;
+    float center[3];
+    float inverse_waist_squared;
+} dp_injection;
+
+/* Source::injections(grid, time) for every source of the scene, in order,
+ * two per source (src/source.rs:280-403). `out` holds 2·source_count. */
+int dp_scene_injections(const dp_scene *scene, float time, dp_injection *out,
+                        size_t capacity);
+
+/* ------------------------------------------------------------------------ */
+/* Device engine: the counterpart of gpu::Simulation (src/gpu.rs:159-799). */
+/* ------------------------------------------------------------------------ */
+typedef struct dp_engine dp_engine;
+typedef struct dp_sim dp_sim;
+
+/* headless_context (src/gpu.rs:805-817): opens HIP device `device`. */
+int dp_engine_create(int device, dp_engine **out);
+void dp_engine_destroy(dp_engine *engine);
+/* Name of the device's ISA, e.g. "gfx950". */
+int dp_engine_arch(dp_engine *engine, char *out, size_t capacity);
+
+/* gpu::Simulation::new (src/gpu.rs:190-340). Validates the grid, resolves
+ * the scene on the host (materials, coefficients, absorber, geometry,
+ * apodization weights) and uploads it; fields start at zero. */
+int dp_sim_create(dp_engine *engine, const dp_scene *scene, dp_sim **out);
+/* Drop (src/gpu.rs:779-799): waits, then frees every device buffer. */
+void dp_sim_destroy(dp_sim *sim);
+
+/* grid().extent, step_count, time (src/gpu.rs:353-364), field_bytes and
+ * bytes_per_step (src/gpu.rs:377-388: 12·N and 48·N). */
+int dp_sim_extent(const dp_sim *sim, uint32_t out[3]);
+uint64_t dp_sim_step_count(const dp_sim *sim);
+float dp_sim_time(const dp_sim *sim);
+float dp_sim_time_step(const dp_sim *sim);
+uint64_t dp_sim_field_bytes(const dp_sim *sim);
+uint64_t dp_sim_bytes_per_step(const dp_sim *sim);
+int dp_sim_is_reversible(const dp_sim *sim);
+
+/* advance_by (src/gpu.rs:393-484): enqueues `steps` full steps on the
+ * sim's stream and returns without waiting. */
+int dp_sim_advance(dp_sim *sim, uint64_t steps);
+/* reverse_by (src/gpu.rs:499-576): DP_ERR_LOSSY / DP_ERR_PAST_START. */
+int dp_sim_reverse(dp_sim *sim, uint64_t steps);
+/* wait (src/gpu.rs:616-622); timeout_ms 0 means the reference's 120 s. */
+int dp_sim_wait(dp_sim *sim, uint32_t timeout_ms);
+/* reset (src/gpu.rs:625-637): zero E, H, intensity; step = 0. */
+int dp_sim_reset(dp_sim *sim);
+
+/* set_accumulate_intensity / accumulated_steps (src/gpu.rs:584-594). The
+ * device accumulator is f64 per cell; each step adds the f32 sum
+ * ((Ex²+Ey²)+Ez²) exactly as cpu.rs:186-196 forms it. */
+int dp_sim_set_accumulate_intensity(dp_sim *sim, int on);
+uint64_t dp_sim_accumulated_steps(const dp_sim *sim);
+
+/* read_electric / read_magnetic (src/gpu.rs:644-656): 3·N floats,
+ * component-major, x fastest. Synchronous. */
+int dp_sim_read_electric(dp_sim *sim, float *out, size_t count);
+int dp_sim_read_magnetic(dp_sim *sim, float *out, size_t count);
+/* read_intensity (src/gpu.rs:702-705): N floats (the f64 sums rounded). */
+int dp_sim_read_intensity(dp_sim *sim, float *out, size_t count);
+int dp_sim_read_intensity_f64(dp_sim *sim, double *out, size_t count);
+
+/* Steppable::restore (src/gpu.rs:756-762) and seeding: uploads E and H
+ * (3·N floats each), sets the step, clears the intensity sum. */
+int dp_sim_write_fields(dp_sim *sim, const float *electric,
+                        const float *magnetic, size_t count, uint64_t step);
+
+/* cpu::Simulation::energy (src/cpu.rs:654-687) computed on the device:
+ * per-component volumes from the primary/dual widths, f64 accumulation. */
+int dp_sim_energy(dp_sim *sim, double *electric, double *magnetic);
+
+/* electric_buffer / magnetic_buffer / intensity_buffer (src/gpu.rs:
+ * 367-374, 597-599): device pointers, borrowed. The engine ping-pongs
+ * E and H between two allocations each, so the pointer names the buffer
+ * holding the current state and stays valid until the next advance,
+ * reverse, reset or write. Intensity is f64. */
+enum { DP_BUFFER_ELECTRIC = 0, DP_BUFFER_MAGNETIC = 1, DP_BUFFER_INTENSITY = 2,
+       DP_BUFFER_MATERIAL_INDEX = 3 };
+int dp_sim_device_ptr(dp_sim *sim, int which, void **out);
+
+/* Kernel path for the forward step. FUSED (default): one z-marching pass
+ * per step doing H, H-injections, E, E-injections and intensity.
+ * TWO_PASS: the reference's dispatch shape (H kernel, injections, E kernel,
+ * injections, intensity). Both give bit-identical fields. */
+enum { DP_KERNEL_FUSED = 0, DP_KERNEL_TWO_PASS = 1 };
+int dp_sim_set_kernel(dp_sim *sim, int kernel);
+/* Fused-kernel tile: rows per wave (y) — 0 selects the tuned default. */
+int dp_sim_set_tile(dp_sim *sim, int rows);
+
+/* HIP-event timing on the sim's stream: record event `slot` (0..7) now;
+ * elapsed milliseconds between two recorded slots (waits for the later). */
+int dp_sim_record_event(dp_sim *sim, int slot);
+int dp_sim_elapsed_ms(dp_sim *sim, int from_slot, int to_slot, float *ms);
+
+/* Forward-step launches enqueued so far (fused or H-kernel launches). */
+uint64_t dp_sim_launch_count(const dp_sim *sim);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* DIAPHANE_AMD_H */

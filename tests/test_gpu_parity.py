@@ -1,0 +1,247 @@
+"""The device kernels against the CPU oracle — tests/parity.rs, ported.
+
+The reference holds its WGSL path to 1e-4 of the field peak because its
+shader compiler may contract FMAs. This engine compiles with
+-ffp-contract=off and IEEE division and evaluates every cell in the oracle's
+operation order, so the bar here is bit equality (max|Δ| == 0); the
+1e-5-of-peak bar is checked as well and reported.
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+import oracle
+from diaphane_amd import (AXES, Axis, Boundary, Extent, Grid, Material, Refinement, Scene, Shape, Source,
+                          Timeline, Waveform)
+from diaphane_amd.gpu import KERNEL_FUSED, KERNEL_TWO_PASS, Simulation
+
+from conftest import assert_field_parity
+
+pytestmark = pytest.mark.gpu
+
+KERNELS = [KERNEL_FUSED, KERNEL_TWO_PASS]
+TOLERANCE = 1e-5  # of the all-component peak
+
+
+def run_pair(context, scene, steps, kernel):
+    reference = oracle.Simulation(scene)
+    candidate = Simulation(context, scene)
+    candidate.set_kernel(kernel)
+    reference.advance_by(steps)
+    candidate.advance_by(steps)
+    return reference, candidate
+
+
+def assert_parity(reference, candidate, steps, exact=True):
+    assert reference.step_count() == candidate.step_count()
+    assert reference.is_finite(), "the oracle diverged"
+    e = candidate.read_electric()
+    h = candidate.read_magnetic()
+    for name, ref_all, cand in (("E", reference.electric_all, e), ("H", reference.magnetic_all, h)):
+        refs = [ref_all[a * reference.cells:(a + 1) * reference.cells] for a in range(3)]
+        cands = [candidate.component(cand, a) for a in AXES]
+        assert_field_parity(name, refs, cands, steps, TOLERANCE)
+        if exact:
+            diff = np.flatnonzero(ref_all.view(np.uint32) != cand.view(np.uint32))
+            assert diff.size == 0, (
+                f"{name}: {diff.size} cells differ in their bits after {steps} steps; first at "
+                f"{diff[0]}: oracle {ref_all[diff[0]]!r} vs device {cand[diff[0]]!r}")
+
+
+@pytest.mark.parametrize("kernel", KERNELS)
+def test_free_space_with_an_absorbing_boundary(context, kernel):
+    # tests/parity.rs:124-134
+    scene = Scene.photon(Extent(96, 40, 36))
+    reference, candidate = run_pair(context, scene, 150, kernel)
+    assert_parity(reference, candidate, 150)
+
+
+@pytest.mark.parametrize("kernel", KERNELS)
+def test_a_point_dipole_in_a_perfectly_conducting_box(context, kernel):
+    # tests/parity.rs:136-141
+    scene = Scene.cavity(Extent(40, 36, 32))
+    reference, candidate = run_pair(context, scene, 500, kernel)
+    assert_parity(reference, candidate, 500)
+
+
+def slab_and_conductor_scene():
+    scene = Scene.empty(Extent(44, 40, 40), 1e-3)
+    frequency = scene.grid.frequency_for_resolution(24.0)
+    glass = scene.materials.push(Material.refractive(1.5))
+    lossy = scene.materials.push(Material.matched_lossy(2.0, 0.05))
+    metal = scene.materials.push(Material.PERFECT_CONDUCTOR)
+    scene.shapes.append(Shape.Slab(Axis.X, 6e-3, 8e-3, glass))
+    scene.shapes.append(Shape.Sphere([-8e-3, 0.0, 0.0], 5e-3, lossy))
+    scene.shapes.append(Shape.Block([14e-3, 0.0, 0.0], [4e-3, 12e-3, 12e-3], metal))
+    scene.sources.append(Source.point([-10e-3, 0.0, 0.0], Axis.Z, Waveform.ricker(frequency)))
+    scene.validate()
+    return scene
+
+
+@pytest.mark.parametrize("kernel", KERNELS)
+def test_a_dielectric_slab_and_a_conductor(context, kernel):
+    # tests/parity.rs:143-183
+    scene = slab_and_conductor_scene()
+    reference, candidate = run_pair(context, scene, 200, kernel)
+    assert_parity(reference, candidate, 200)
+
+
+def several_sources_scene():
+    scene = Scene.empty(Extent.cube(40), 1e-3).with_boundary(Boundary.absorbing(8, 1e-6))
+    frequency = scene.grid.frequency_for_resolution(20.0)
+    scene.sources.append(Source.point([0.0] * 3, Axis.Z, Waveform.ricker(frequency)))
+    scene.sources.append(Source.point([0.0] * 3, Axis.X, Waveform.ricker(frequency)).with_amplitude(-0.5))
+    scene.sources.append(Source.sheet(Axis.Y, -8e-3, 6e-3, Axis.Z, Waveform.gaussian_pulse(frequency, 3.0)))
+    scene.sources.append(Source.point_moving([-6e-3, 2e-3, 0.0], [2.7e8, 0.0, 0.0], Axis.Z,
+                                             Waveform.ricker(frequency)))
+    return scene
+
+
+@pytest.mark.parametrize("kernel", KERNELS)
+def test_several_sources_at_once(context, kernel):
+    # tests/parity.rs:185-219: overlapping sources and a mover that exits.
+    reference, candidate = run_pair(context, several_sources_scene(), 250, kernel)
+    assert_parity(reference, candidate, 250)
+
+
+def graded_scene():
+    grid = Grid.graded([48e-3, 40e-3, 36e-3], 1e3, [
+        Refinement.across(Axis.X, 6e-3, 8e-3, 0.25e-3),
+        Refinement.across(Axis.Y, -4e-3, 6e-3, 0.5e-3),
+    ])
+    scene = Scene.on_grid(grid).with_boundary(Boundary.absorbing(8, 1e-6))
+    frequency = scene.grid.frequency_for_resolution(24.0)
+    glass = scene.materials.push(Material.refractive(1.5))
+    scene.shapes.append(Shape.Slab(Axis.X, 6e-3, 2e-3, glass))
+    scene.sources.append(Source.point([0.0, -4e-3, 0.0], Axis.Z, Waveform.ricker(frequency)))
+    scene.validate()
+    return scene
+
+
+@pytest.mark.parametrize("kernel", KERNELS)
+def test_a_graded_grid_is_graded_the_same_way_on_the_gpu(context, kernel):
+    # tests/parity.rs:221-265
+    reference, candidate = run_pair(context, graded_scene(), 280, kernel)
+    assert_parity(reference, candidate, 280)
+
+
+def plane_wave_scene():
+    scene = Scene.empty(Extent(96, 36, 36), 1e-3)
+    frequency = scene.grid.frequency_for_resolution(20.0)
+    glass = scene.materials.push(Material.refractive(1.5))
+    scene.shapes.append(Shape.Sphere([8e-3, 0.0, 0.0], 6e-3, glass))
+    scene.sources.append(Source.plane_wave(Axis.X, -0.020, Axis.Z, Waveform.gaussian_pulse(frequency, 3.0)))
+    scene.validate()
+    return scene
+
+
+@pytest.mark.parametrize("kernel", KERNELS)
+def test_a_plane_wave_agrees_between_the_solvers(context, kernel):
+    # tests/parity.rs:267-289: the one source that injects into H.
+    reference, candidate = run_pair(context, plane_wave_scene(), 350, kernel)
+    assert_parity(reference, candidate, 350)
+
+
+@pytest.mark.parametrize("kernel", KERNELS)
+def test_time_reversal_matches_between_the_solvers(context, kernel):
+    # tests/parity.rs:291-346
+    scene = Scene.empty(Extent(40, 36, 36), 1e-3).with_boundary(Boundary.PEC)
+    frequency = scene.grid.frequency_for_resolution(24.0)
+    glass = scene.materials.push(Material.refractive(2.0))
+    scene.shapes.append(Shape.Sphere([4e-3, 0.0, 0.0], 6e-3, glass))
+    scene.sources.append(Source.point([-12e-3, 0.0, 0.0], Axis.Z, Waveform.ricker(frequency)))
+    scene.validate()
+    assert scene.is_reversible()
+    reference, candidate = run_pair(context, scene, 220, kernel)
+    reference.reverse_by(140)
+    candidate.reverse_by(140)
+    assert_parity(reference, candidate, 360)
+
+
+@pytest.mark.parametrize("kernel", KERNELS)
+def test_the_intensity_accumulators_agree(context, kernel):
+    # tests/parity.rs:348-383, plus the f64 sums compared bit for bit.
+    scene = Scene.photon(Extent.cube(32))
+    reference, candidate = run_pair(context, scene, 60, kernel)
+    reference.set_accumulate_intensity(True)
+    candidate.set_accumulate_intensity(True)
+    reference.advance_by(90)
+    candidate.advance_by(90)
+    assert reference.accumulated_steps() == 90
+    assert candidate.accumulated_steps() == 90
+    host = reference.intensity()
+    device = candidate.read_intensity()
+    peak = float(host.max())
+    assert peak > 0.0
+    assert float(np.max(np.abs(host - device))) < 1e-4 * peak
+    assert np.array_equal(reference.intensity_f64(), candidate.read_intensity_f64())
+
+
+def test_resetting_clears_the_device_buffers(context):
+    # tests/parity.rs:385-404
+    simulation = Simulation(context, Scene.photon(Extent.cube(32)))
+    simulation.advance_by(120)
+    assert np.any(simulation.read_electric() != 0.0)
+    simulation.reset()
+    assert simulation.step_count() == 0
+    assert np.all(simulation.read_electric() == 0.0)
+    assert np.all(simulation.read_magnetic() == 0.0)
+
+
+def test_a_timeline_scrubs_the_gpu_solver(context):
+    # tests/parity.rs:406-445
+    scene = Scene.photon(Extent.cube(32))
+    reference = Simulation(context, scene)
+    reference.advance_by(70)
+    expected = reference.read_electric()
+    simulation = Simulation(context, scene)
+    timeline = Timeline(25, 8)
+    for _ in range(120):
+        simulation.advance_by(1)
+        timeline.observe(simulation)
+    assert timeline.keyframe_count() > 1
+    timeline.seek(simulation, 70)
+    assert simulation.step_count() == 70
+    scrubbed = simulation.read_electric()
+    assert float(np.abs(expected).max()) > 0.0
+    assert np.array_equal(expected, scrubbed)
+
+
+def test_fused_tiles_agree_bit_for_bit(context):
+    # The race canary: every tile shape and both kernels give the same bits.
+    scene = several_sources_scene()
+    results = []
+    for kernel, rows in [(KERNEL_TWO_PASS, 0), (KERNEL_FUSED, 4), (KERNEL_FUSED, 6), (KERNEL_FUSED, 8),
+                         (KERNEL_FUSED, 10), (KERNEL_FUSED, 12)]:
+        sim = Simulation(context, scene)
+        sim.set_kernel(kernel)
+        if rows:
+            sim.set_tile(rows)
+        sim.advance_by(120)
+        results.append((sim.read_electric(), sim.read_magnetic()))
+    for e, h in results[1:]:
+        assert np.array_equal(e, results[0][0])
+        assert np.array_equal(h, results[0][1])
+
+
+def test_energy_matches_the_oracle(context):
+    scene = slab_and_conductor_scene()
+    reference, candidate = run_pair(context, scene, 120, KERNEL_FUSED)
+    ue, uh = reference.energy().electric, reference.energy().magnetic
+    got = candidate.energy()
+    assert ue > 0 and uh > 0
+    assert abs(got.electric - ue) <= 1e-12 * ue
+    assert abs(got.magnetic - uh) <= 1e-12 * uh
+
+
+def test_restore_then_advance_matches(context):
+    # Steppable::restore seeds the device; continuing must match the oracle.
+    scene = plane_wave_scene()
+    reference = oracle.Simulation(scene)
+    reference.advance_by(100)
+    candidate = Simulation(context, scene)
+    candidate.write_fields(reference.electric_all, reference.magnetic_all, reference.step_count())
+    reference.advance_by(50)
+    candidate.advance_by(50)
+    assert_parity(reference, candidate, 150)
