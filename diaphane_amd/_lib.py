@@ -45,6 +45,7 @@ def _declare(lib: ctypes.CDLL) -> None:
         "dp_scene_validate": (c_int, [POINTER(DpScene)]),
         "dp_scene_material_indices": (c_int, [POINTER(DpScene), POINTER(c_uint32), c_size_t]),
         "dp_scene_injections": (c_int, [POINTER(DpScene), c_float, POINTER(DpInjection), c_size_t]),
+        "dp_scene_tables": (c_int, [POINTER(DpScene), POINTER(c_float), c_size_t, POINTER(c_float), c_size_t]),
         "dp_engine_create": (c_int, [c_int, POINTER(c_void_p)]),
         "dp_engine_destroy": (None, [c_void_p]),
         "dp_engine_arch": (c_int, [c_void_p, ctypes.c_char_p, c_size_t]),
@@ -71,10 +72,11 @@ def _declare(lib: ctypes.CDLL) -> None:
         "dp_sim_energy": (c_int, [sim, POINTER(c_double), POINTER(c_double)]),
         "dp_sim_device_ptr": (c_int, [sim, c_int, POINTER(c_void_p)]),
         "dp_sim_set_kernel": (c_int, [sim, c_int]),
-        "dp_sim_set_tile": (c_int, [sim, c_int]),
+        "dp_sim_set_tile": (c_int, [sim, c_int, c_int, c_int]),
         "dp_sim_record_event": (c_int, [sim, c_int]),
         "dp_sim_elapsed_ms": (c_int, [sim, c_int, c_int, POINTER(c_float)]),
         "dp_sim_launch_count": (c_uint64, [sim]),
+        "dp_sim_stream_probe": (c_int, [sim, c_int, POINTER(c_double)]),
     }
     for name, (restype, argtypes) in table.items():
         fn = getattr(lib, name)

@@ -118,7 +118,8 @@ struct dp_sim {
     uint64_t accumulated = 0;
     int kernel = DP_KERNEL_FUSED;
     int rows = 0;  // 0: tuned default
-    uint32_t depth = 0;
+    uint32_t depth = 0;  // 0: tuned default
+    int tile_flags = DP_TILE_XCD_REMAP;
     uint64_t launches = 0;
     hipEvent_t events[8] = {};
 
@@ -151,7 +152,7 @@ struct dp_sim {
 
     int effective_rows() const {
         if (rows > 0) return rows;
-        return 8;
+        return kernel == DP_KERNEL_FUSED_OVERLAP ? 6 : 6;
     }
 
     void ensure_intensity() {
@@ -217,14 +218,14 @@ struct dp_sim {
               "hipMemcpyAsync");
     }
 
-    void fill_tiles(StepArgs &a, int r) const {
+    void fill_tiles(StepArgs &a, int r, uint32_t owned_x) const {
         const uint32_t nx = grid.extent[0], ny = grid.extent[1], nz = grid.extent[2];
-        a.tiles_x = (nx + kOwnedX - 1) / kOwnedX;
+        a.tiles_x = (nx + owned_x - 1) / owned_x;
         a.tiles_y = (ny + (r - 2) - 1) / (r - 2);
-        a.depth = depth ? depth : std::min<uint32_t>(nz, 64);
+        a.depth = std::min<uint32_t>(nz, depth ? depth : 32);
         a.tiles_z = (nz + a.depth - 1) / a.depth;
         a.tiles_total = a.tiles_x * a.tiles_y * a.tiles_z;
-        a.xcd_remap = 1;
+        a.xcd_remap = (tile_flags & DP_TILE_XCD_REMAP) ? 1 : 0;
     }
 
     void advance(uint64_t steps) {
@@ -237,14 +238,19 @@ struct dp_sim {
         const int r = effective_rows();
         for (uint64_t s = 0; s < steps; ++s) {
             const auto &t = table[s];
-            if (kernel == DP_KERNEL_FUSED) {
+            if (kernel == DP_KERNEL_FUSED || kernel == DP_KERNEL_FUSED_OVERLAP) {
                 StepArgs a = args(current, 1 - current);
                 a.injections = device_inj + t.offset;
                 a.n_magnetic = t.magnetic;
                 a.n_electric = t.electric;
                 a.intensity = accumulating ? intensity.as<double>() : nullptr;
-                fill_tiles(a, r);
-                launch_fused(a, r, stream);
+                if (kernel == DP_KERNEL_FUSED) {
+                    fill_tiles(a, r, kAlignedX);
+                    launch_fused_aligned(a, r, (tile_flags & DP_TILE_NONTEMPORAL) != 0, stream);
+                } else {
+                    fill_tiles(a, r, kOwnedX);
+                    launch_fused(a, r, stream);
+                }
                 current = 1 - current;
             } else {
                 StepArgs a = args(current, current);
@@ -353,6 +359,41 @@ int dp_scene_material_indices(const dp_scene *scene, uint32_t *out, size_t count
         Grid g = Grid::resolve(scene->grid);
         if (count != g.cells()) fail(DP_ERR_SIZE, "index buffer length does not match the grid");
         rasterize(*scene, g, out);
+    });
+}
+
+int dp_scene_tables(const dp_scene *scene, float *coefficients, size_t coefficient_capacity,
+                    float *axis_table, size_t axis_capacity) {
+    return guarded([&] {
+        check_scene_pointers(scene);
+        Grid g = Grid::resolve(scene->grid);
+        if (coefficients) {
+            if (coefficient_capacity < 4 * size_t(scene->material_count))
+                fail(DP_ERR_SIZE, "coefficient buffer too small");
+            const float dt = g.time_step();
+            for (uint32_t m = 0; m < scene->material_count; ++m) {
+                Coefficients c = material_coefficients(scene->materials[m], dt);
+                coefficients[4 * m + 0] = c.electric_gain;
+                coefficients[4 * m + 1] = c.electric_loss;
+                coefficients[4 * m + 2] = c.magnetic_gain;
+                coefficients[4 * m + 3] = c.magnetic_loss;
+            }
+        }
+        if (axis_table) {
+            const size_t span = size_t(g.extent[0]) + g.extent[1] + g.extent[2];
+            if (axis_capacity < 4 * span) fail(DP_ERR_SIZE, "axis table buffer too small");
+            Absorber absorber = Absorber::build(g, *scene);
+            size_t at = 0;
+            for (int a = 0; a < 3; at += g.extent[a], ++a) {
+                const auto gh = g.gains(g.axis[a].primary), ge = g.gains(g.axis[a].dual);
+                for (uint32_t i = 0; i < g.extent[a]; ++i) {
+                    axis_table[kGainH * span + at + i] = gh[i];
+                    axis_table[kGainE * span + at + i] = ge[i];
+                    axis_table[kLossInt * span + at + i] = absorber.integer[a][i];
+                    axis_table[kLossHalf * span + at + i] = absorber.half[a][i];
+                }
+            }
+        }
     });
 }
 
@@ -659,18 +700,43 @@ int dp_sim_device_ptr(dp_sim *sim, int which, void **out) {
 int dp_sim_set_kernel(dp_sim *sim, int kernel) {
     return guarded([&] {
         if (!sim) fail(DP_ERR_INVALID, "null sim");
-        if (kernel != DP_KERNEL_FUSED && kernel != DP_KERNEL_TWO_PASS) fail(DP_ERR_INVALID, "unknown kernel");
+        if (kernel != DP_KERNEL_FUSED && kernel != DP_KERNEL_TWO_PASS && kernel != DP_KERNEL_FUSED_OVERLAP)
+            fail(DP_ERR_INVALID, "unknown kernel");
         sim->kernel = kernel;
     });
 }
 
-int dp_sim_set_tile(dp_sim *sim, int rows) {
+int dp_sim_set_tile(dp_sim *sim, int rows, int depth, int flags) {
     return guarded([&] {
         if (!sim) fail(DP_ERR_INVALID, "null sim");
         static const int allowed[] = {0, 4, 6, 8, 10, 12};
         if (std::find(std::begin(allowed), std::end(allowed), rows) == std::end(allowed))
             fail(DP_ERR_INVALID, "rows must be one of 4, 6, 8, 10, 12 (or 0 for the default)");
+        if (depth < 0) fail(DP_ERR_INVALID, "depth must be non-negative");
         sim->rows = rows;
+        sim->depth = static_cast<uint32_t>(depth);
+        if (flags >= 0) sim->tile_flags = flags & (DP_TILE_XCD_REMAP | DP_TILE_NONTEMPORAL);
+    });
+}
+
+int dp_sim_stream_probe(dp_sim *sim, int iterations, double *gbs) {
+    return guarded([&] {
+        if (!sim || !gbs || iterations <= 0) fail(DP_ERR_INVALID, "bad argument");
+        const int from = sim->current, to = 1 - sim->current;
+        auto launch = [&] {
+            launch_stream_probe(sim->e[from].as<float>(), sim->h[from].as<float>(), sim->material.as<uint8_t>(),
+                                sim->e[to].as<float>(), sim->h[to].as<float>(), sim->cells, sim->stream);
+        };
+        launch();
+        check(hipGetLastError(), "kernel launch");
+        check(hipEventRecord(sim->events[6], sim->stream), "hipEventRecord");
+        for (int i = 0; i < iterations; ++i) launch();
+        check(hipEventRecord(sim->events[7], sim->stream), "hipEventRecord");
+        check(hipEventSynchronize(sim->events[7]), "hipEventSynchronize");
+        float ms = 0.0f;
+        check(hipEventElapsedTime(&ms, sim->events[6], sim->events[7]), "hipEventElapsedTime");
+        const double bytes = 49.0 * double(sim->cells / 4 * 4) * iterations;
+        *gbs = bytes / (double(ms) / 1e3) / 1e9;
     });
 }
 

@@ -62,7 +62,11 @@ struct StepArgs {
 constexpr int kLanes = 64;
 constexpr int kOwnedX = kLanes - 2;
 
+// 62-wide tiles whose halo columns are ordinary lanes (the first fused form).
 void launch_fused(const StepArgs &args, int rows, hipStream_t stream);
+// 64-aligned tiles; halo columns fetched by lanes 0 and 63.
+constexpr int kAlignedX = 64;
+void launch_fused_aligned(const StepArgs &args, int rows, bool nontemporal, hipStream_t stream);
 void launch_update_h(const StepArgs &args, int direction, hipStream_t stream);
 void launch_update_e(const StepArgs &args, int direction, hipStream_t stream);
 void launch_inject(float *field, uint64_t cells, uint32_t nx, uint32_t ny, const InjDesc &desc,
@@ -73,5 +77,9 @@ uint32_t launch_energy(const float *e, const float *h, const uint8_t *material,
                        const float2 *eps_mu, const float *widths, uint32_t nx, uint32_t ny,
                        uint32_t nz, double *partials, hipStream_t stream);
 uint32_t energy_blocks(uint32_t nx, uint32_t ny, uint32_t nz);
+// Streams the step's access mix (E, H, material in; E, H out) with no
+// stencil: the practical bandwidth ceiling for the step's byte count.
+void launch_stream_probe(const float *e, const float *h, const uint8_t *material, float *e_out, float *h_out,
+                         uint64_t cells, hipStream_t stream);
 
 }  // namespace dpa

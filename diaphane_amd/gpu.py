@@ -19,6 +19,9 @@ from .timeline import Snapshot
 
 KERNEL_FUSED = 0
 KERNEL_TWO_PASS = 1
+KERNEL_FUSED_OVERLAP = 2
+TILE_XCD_REMAP = 1
+TILE_NONTEMPORAL = 2
 
 BUFFER_ELECTRIC = 0
 BUFFER_MAGNETIC = 1
@@ -197,8 +200,9 @@ class Simulation:
     def set_kernel(self, kernel: int) -> None:
         check(lib().dp_sim_set_kernel(self._handle, kernel))
 
-    def set_tile(self, rows: int) -> None:
-        check(lib().dp_sim_set_tile(self._handle, rows))
+    def set_tile(self, rows: int = 0, depth: int = 0, flags: int = -1) -> None:
+        """Fused-kernel tiling; flags: 1 = XCD-contiguous tiles, 2 = non-temporal stores."""
+        check(lib().dp_sim_set_tile(self._handle, rows, depth, flags))
 
     def record_event(self, slot: int) -> None:
         check(lib().dp_sim_record_event(self._handle, slot))
@@ -207,6 +211,12 @@ class Simulation:
         ms = c_float()
         check(lib().dp_sim_elapsed_ms(self._handle, start, end, byref(ms)))
         return float(ms.value)
+
+    def stream_probe(self, iterations: int = 10) -> float:
+        """GB/s of the step's access mix streamed with no stencil."""
+        gbs = c_double()
+        check(lib().dp_sim_stream_probe(self._handle, iterations, byref(gbs)))
+        return gbs.value
 
     def launch_count(self) -> int:
         return int(lib().dp_sim_launch_count(self._handle))

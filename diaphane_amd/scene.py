@@ -688,6 +688,20 @@ class Scene:
                                               out.size))
         return out
 
+    def tables(self) -> Tuple[np.ndarray, np.ndarray]:
+        """(coefficients [n_materials, 4], axis table [4, nx+ny+nz]) as the engine uploads them."""
+        from ._lib import check, lib
+
+        c = self.to_c()
+        extent = self.grid.extent
+        span = extent.x + extent.y + extent.z
+        coefficients = np.zeros(4 * len(self.materials), dtype=np.float32)
+        axis = np.zeros(4 * span, dtype=np.float32)
+        fp = ctypes.POINTER(c_float)
+        check(lib().dp_scene_tables(ctypes.byref(c), coefficients.ctypes.data_as(fp), coefficients.size,
+                                    axis.ctypes.data_as(fp), axis.size))
+        return coefficients.reshape(-1, 4), axis.reshape(4, span)
+
     def injections(self, time: float) -> List[DpInjection]:
         """Source::injections for every source, in order (two each)."""
         from ._lib import check, lib

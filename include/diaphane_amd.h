@@ -179,6 +179,18 @@ typedef struct dp_injection {
     float inverse_waist_squared;
 } dp_injection;
 
+/* The tables the kernels read, as the engine resolves them on the host:
+ * per material (electric_gain, electric_loss, magnetic_gain, magnetic_loss)
+ * (Material::coefficients, src/material.rs:100-122; 4·material_count
+ * floats), and per axis 4·(nx+ny+nz) floats laid out
+ * [primary gains x|y|z][dual gains x|y|z][integer absorber x|y|z]
+ * [half absorber x|y|z] (Grid::packed_geometry sections 0-1,
+ * src/grid.rs:666-678; AbsorbingProfile::packed, src/boundary.rs:203-212).
+ * Either output may be NULL. */
+int dp_scene_tables(const dp_scene *scene, float *coefficients,
+                    size_t coefficient_capacity, float *axis_table,
+                    size_t axis_capacity);
+
 /* Source::injections(grid, time) for every source of the scene, in order,
  * two per source (src/source.rs:280-403). `out` holds 2·source_count. */
 int dp_scene_injections(const dp_scene *scene, float time, dp_injection *out,
@@ -259,15 +271,29 @@ int dp_sim_device_ptr(dp_sim *sim, int which, void **out);
  * per step doing H, H-injections, E, E-injections and intensity.
  * TWO_PASS: the reference's dispatch shape (H kernel, injections, E kernel,
  * injections, intensity). Both give bit-identical fields. */
-enum { DP_KERNEL_FUSED = 0, DP_KERNEL_TWO_PASS = 1 };
+enum { DP_KERNEL_FUSED = 0, DP_KERNEL_TWO_PASS = 1, DP_KERNEL_FUSED_OVERLAP = 2 };
+/* DP_KERNEL_FUSED uses 64-aligned tiles whose x halo is fetched by the edge
+ * lanes; DP_KERNEL_FUSED_OVERLAP uses 62-wide tiles whose halo columns are
+ * ordinary lanes. */
+enum { DP_TILE_XCD_REMAP = 1, DP_TILE_NONTEMPORAL = 2 };
 int dp_sim_set_kernel(dp_sim *sim, int kernel);
-/* Fused-kernel tile: rows per wave (y) — 0 selects the tuned default. */
-int dp_sim_set_tile(dp_sim *sim, int rows);
+/* Fused-kernel tiling: rows per wave along y (4, 6, 8, 10 or 12), z-slab
+ * depth in planes, and flags: DP_TILE_XCD_REMAP renumbers workgroups so each
+ * XCD walks a contiguous run of tiles, DP_TILE_NONTEMPORAL streams the field
+ * stores past the caches. 0 for rows/depth selects the tuned default;
+ * flags < 0 keeps the current flags. Results never depend on any of it. */
+int dp_sim_set_tile(dp_sim *sim, int rows, int depth, int flags);
 
 /* HIP-event timing on the sim's stream: record event `slot` (0..7) now;
  * elapsed milliseconds between two recorded slots (waits for the later). */
 int dp_sim_record_event(dp_sim *sim, int slot);
 int dp_sim_elapsed_ms(dp_sim *sim, int from_slot, int to_slot, float *ms);
+
+/* Bandwidth (GB/s) of the step's access mix — E, H and the u8 material
+ * read, E and H written, 49 bytes per cell — streamed with no stencil on
+ * this sim's own buffers, averaged over `iterations` launches. The state is
+ * left unchanged. The practical ceiling for the step's byte count. */
+int dp_sim_stream_probe(dp_sim *sim, int iterations, double *gbs);
 
 /* Forward-step launches enqueued so far (fused or H-kernel launches). */
 uint64_t dp_sim_launch_count(const dp_sim *sim);

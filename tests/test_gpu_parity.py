@@ -14,13 +14,13 @@ import pytest
 import oracle
 from diaphane_amd import (AXES, Axis, Boundary, Extent, Grid, Material, Refinement, Scene, Shape, Source,
                           Timeline, Waveform)
-from diaphane_amd.gpu import KERNEL_FUSED, KERNEL_TWO_PASS, Simulation
+from diaphane_amd.gpu import KERNEL_FUSED, KERNEL_FUSED_OVERLAP, KERNEL_TWO_PASS, Simulation
 
 from conftest import assert_field_parity
 
 pytestmark = pytest.mark.gpu
 
-KERNELS = [KERNEL_FUSED, KERNEL_TWO_PASS]
+KERNELS = [KERNEL_FUSED, KERNEL_TWO_PASS, KERNEL_FUSED_OVERLAP]
 TOLERANCE = 1e-5  # of the all-component peak
 
 
@@ -212,12 +212,16 @@ def test_fused_tiles_agree_bit_for_bit(context):
     # The race canary: every tile shape and both kernels give the same bits.
     scene = several_sources_scene()
     results = []
-    for kernel, rows in [(KERNEL_TWO_PASS, 0), (KERNEL_FUSED, 4), (KERNEL_FUSED, 6), (KERNEL_FUSED, 8),
-                         (KERNEL_FUSED, 10), (KERNEL_FUSED, 12)]:
+    variants = [(KERNEL_TWO_PASS, 0, 0, 1)]
+    for kernel in (KERNEL_FUSED, KERNEL_FUSED_OVERLAP):
+        for rows in (4, 6, 8, 10, 12):
+            variants.append((kernel, rows, 0, 1))
+    variants += [(KERNEL_FUSED, 6, 7, 3), (KERNEL_FUSED, 4, 13, 2), (KERNEL_FUSED, 8, 40, 0)]
+    for kernel, rows, depth, flags in variants:
         sim = Simulation(context, scene)
         sim.set_kernel(kernel)
         if rows:
-            sim.set_tile(rows)
+            sim.set_tile(rows, depth, flags)
         sim.advance_by(120)
         results.append((sim.read_electric(), sim.read_magnetic()))
     for e, h in results[1:]:
