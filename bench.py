@@ -67,6 +67,8 @@ def main() -> int:
     parser.add_argument("--rows", type=int, default=0, help="fused-kernel tile rows (0 = default)")
     parser.add_argument("--no-cpu", action="store_true", help="skip the CPU-oracle baseline")
     parser.add_argument("--cpu-seconds", type=float, default=10.0)
+    parser.add_argument("--intensity-from", type=int, default=None,
+                        help="step at which |E|² accumulation starts (-1: never; default: the workload's)")
     args = parser.parse_args()
 
     if args.gpus != 1:
@@ -82,12 +84,17 @@ def main() -> int:
     accumulate_from = work.accumulate_from
     if accumulate_from is not None and args.steps is not None:
         accumulate_from = steps // 2
+    if args.intensity_from is not None:
+        accumulate_from = None if args.intensity_from < 0 else min(args.intensity_from, steps)
 
     context = headless_context()
     sim = Simulation(context, work.scene)
     if args.rows:
         sim.set_tile(args.rows)
     cells = sim.cells
+
+    # The step's access mix streamed with no stencil: the practical ceiling.
+    probe_gbs = sim.stream_probe(5)
 
     if args.warmup:
         sim.advance_by(args.warmup)
@@ -120,6 +127,14 @@ def main() -> int:
     achieved = (bytes_a + bytes_b) / kernel_s / 1e9
     value = cells * steps / wall
 
+    traffic = None
+    traffic_file = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
+                                f"k_fused_aligned_{work.name}.json")
+    if os.path.exists(traffic_file):
+        with open(traffic_file) as handle:
+            measured = json.load(handle)
+        traffic = measured.get("traffic_bytes_per_launch")
+
     result = {
         "metric": "Yee cell-updates/s & %HBM roofline",
         "value": value,
@@ -146,8 +161,10 @@ def main() -> int:
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
-            "kernel": "k_fused",
+            "traffic": traffic,
+            "kernel": "k_fused_aligned",
+            "stream_probe_gbs": probe_gbs,
+            "frac_of_stream_probe": achieved / probe_gbs,
             "bytes_per_cell": {"plain": BYTES_PER_CELL, "with_intensity_f64": BYTES_PER_CELL_INTENSITY},
             "avg_launch_ms": {"plain": ms_a / max(phase_a, 1), "with_intensity": ms_b / max(phase_b, 1)},
             "phase_gbs": {

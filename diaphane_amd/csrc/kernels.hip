@@ -459,8 +459,12 @@ __global__ __launch_bounds__(256) void k_fused_aligned(StepArgs p) {
         float hy_lo[R - 1], hz_lo[R - 1];
 #pragma unroll
         for (int r = 0; r < R - 1; ++r) {
-            const float ez_xp = lane == kLanes - 1 ? gz[r] : __shfl_down(ez[r], 1, kLanes);
-            const float ey_xp = lane == kLanes - 1 ? gy[r] : __shfl_down(ey[r], 1, kLanes);
+            // The shuffles run on every lane before the select: a cross-lane
+            // read from a lane that is switched off by a branch is undefined.
+            const float ez_down = __shfl_down(ez[r], 1, kLanes);
+            const float ey_down = __shfl_down(ey[r], 1, kLanes);
+            const float ez_xp = lane == kLanes - 1 ? gz[r] : ez_down;
+            const float ey_xp = lane == kLanes - 1 ? gy[r] : ey_down;
             const float4 c = s_coeff[mat[r]];
             const float mg = c.z, ml = c.w;
             {
