@@ -291,286 +291,7 @@ __global__ __launch_bounds__(256) void k_fused(StepArgs p) {
     }
 }
 
-template <bool kNonTemporal>
-__device__ __forceinline__ void store(float *p, float v) {
-    if (kNonTemporal)
-        __builtin_nontemporal_store(v, p);
-    else
-        *p = v;
-}
-
-// ---------------------------------------------------------------------------
-// Fused step on 64-aligned tiles.
-//
-// Lane l owns x = 64·tx + l, so every row segment a wave loads or stores is
-// one aligned 256-byte run. The x halo comes from two lanes: lane 0 also
-// loads the cell at x0−1 and lane 63 the cell at x0+64 (the neighbouring
-// tiles' edge values, which sit in lines those tiles are streaming through
-// L2 at the same time). Lane 0 recomputes H_new at x0−1 (Hy and Hz, the two
-// components E differences along x); lane 63 only needs E at x0+64.
-//
-// Rows are as in k_fused: R register rows, rows 1..R−2 owned, row 0 a halo
-// whose Hx and Hz E needs, row R−1 a halo whose Ex and Ez H needs. The
-// z-slab and ping-pong discipline are the same.
-// ---------------------------------------------------------------------------
-template <int R, bool kNonTemporal>
-__global__ __launch_bounds__(256) void k_fused_aligned(StepArgs p) {
-    static_assert(R >= 3, "a tile needs a halo row on each side");
-    __shared__ float4 s_coeff[256];
-    for (uint32_t m = threadIdx.x; m < p.material_count; m += blockDim.x) s_coeff[m] = p.coefficients[m];
-    __syncthreads();
-
-    uint32_t block = blockIdx.x;
-    if (p.xcd_remap) block = xcd_contiguous(block, gridDim.x);
-    const uint32_t tile = __builtin_amdgcn_readfirstlane(block * (blockDim.x / kLanes) + threadIdx.x / kLanes);
-    if (tile >= p.tiles_total) return;
-    const int lane = threadIdx.x & (kLanes - 1);
-
-    const uint32_t nx = p.nx, ny = p.ny, nz = p.nz;
-    const uint32_t span = nx + ny + nz;
-    const uint64_t plane = uint64_t(nx) * ny;
-    const uint64_t cells = p.cells;
-    const float *__restrict__ table = p.axis_table;
-    const float *__restrict__ e_in = p.e_in;
-    const float *__restrict__ h_in = p.h_in;
-    float *__restrict__ e_out = p.e_out;
-    float *__restrict__ h_out = p.h_out;
-    const uint8_t *__restrict__ material = p.material;
-    const InjDesc *__restrict__ magnetic_list = p.injections;
-    const InjDesc *__restrict__ electric_list = p.injections + p.n_magnetic;
-    const float *__restrict__ weights = p.weights;
-
-    const uint32_t tx = tile % p.tiles_x;
-    const uint32_t rest = tile / p.tiles_x;
-    const uint32_t ty = rest % p.tiles_y;
-    const uint32_t tz = rest / p.tiles_y;
-    const int x0 = int(tx * kLanes);
-    const int x = x0 + lane;
-    const int y0 = int(ty * (R - 2)) - 1;
-    const uint32_t zb = tz * p.depth;
-    const uint32_t ze = min(zb + p.depth, nz);
-    const uint32_t zs = zb > 0 ? zb - 1 : 0;
-
-    const bool x_in = x < int(nx);
-    // Halo column of this lane: x0−1 for lane 0, x0+64 for lane 63.
-    const int xh = lane == 0 ? x0 - 1 : (lane == kLanes - 1 ? x0 + kLanes : -1);
-    const bool xh_in = xh >= 0 && xh < int(nx);
-
-    float gain_h_x = 0.0f, gain_e_x = 0.0f, loss_int_x = 0.0f, loss_half_x = 0.0f;
-    if (x_in) {
-        gain_h_x = axis_value(table, span, kGainH, x);
-        gain_e_x = axis_value(table, span, kGainE, x);
-        loss_int_x = axis_value(table, span, kLossInt, x);
-        loss_half_x = axis_value(table, span, kLossHalf, x);
-    }
-    // x−1 constants for lane 0's halo H (x0−1).
-    float gain_h_xl = 0.0f, loss_half_xl = 0.0f;
-    if (lane == 0 && xh_in) {
-        gain_h_xl = axis_value(table, span, kGainH, xh);
-        loss_half_xl = axis_value(table, span, kLossHalf, xh);
-    }
-    bool y_in[R];
-    float gain_h_y[R], gain_e_y[R], loss_int_y[R], loss_half_y[R];
-#pragma unroll
-    for (int r = 0; r < R; ++r) {
-        const int y = y0 + r;
-        y_in[r] = y >= 0 && y < int(ny);
-        const uint32_t yc = y_in[r] ? uint32_t(y) : 0;
-        gain_h_y[r] = axis_value(table, span, kGainH, nx + yc);
-        gain_e_y[r] = axis_value(table, span, kGainE, nx + yc);
-        loss_int_y[r] = axis_value(table, span, kLossInt, nx + yc);
-        loss_half_y[r] = axis_value(table, span, kLossHalf, nx + yc);
-    }
-    auto column = [&](int r) -> int64_t { return int64_t(y0 + r) * nx + x; };
-    auto halo_column = [&](int r) -> int64_t { return int64_t(y0 + r) * nx + xh; };
-
-    // E(k) of the owned lanes and of the halo lanes (lane 0: x0−1, 63: x0+64).
-    float ex[R], ey[R], ez[R], gx[R], gy[R], gz[R];
-#pragma unroll
-    for (int r = 0; r < R; ++r) {
-        ex[r] = ey[r] = ez[r] = gx[r] = gy[r] = gz[r] = 0.0f;
-        const uint64_t k0 = uint64_t(zs) * plane;
-        if (x_in && y_in[r]) {
-            const uint64_t i = k0 + column(r);
-            ex[r] = e_in[i];
-            ey[r] = e_in[cells + i];
-            ez[r] = e_in[2 * cells + i];
-        }
-        if (xh_in && y_in[r]) {
-            const uint64_t i = k0 + halo_column(r);
-            gx[r] = e_in[i];
-            gy[r] = e_in[cells + i];
-            gz[r] = e_in[2 * cells + i];
-        }
-    }
-    float hx_prev[R], hy_prev[R];
-#pragma unroll
-    for (int r = 0; r < R; ++r) hx_prev[r] = hy_prev[r] = 0.0f;
-
-    for (uint32_t k = zs; k < ze; ++k) {
-        const uint32_t kz = nx + ny + k;
-        const float gain_h_z = axis_value(table, span, kGainH, kz);
-        const float gain_e_z = axis_value(table, span, kGainE, kz);
-        const float loss_int_z = axis_value(table, span, kLossInt, kz);
-        const float loss_half_z = axis_value(table, span, kLossHalf, kz);
-        const bool next_in = k + 1 < nz;
-        const uint64_t kp = uint64_t(k) * plane, kn = uint64_t(k + 1) * plane;
-
-        float nex[R], ney[R], nez[R], ngx[R], ngy[R], ngz[R];
-        float hx[R - 1], hy[R - 1], hz[R - 1], hgy[R - 1], hgz[R - 1];
-        uint32_t mat[R - 1], hmat[R - 1];
-#pragma unroll
-        for (int r = 0; r < R; ++r) {
-            nex[r] = ney[r] = nez[r] = ngx[r] = ngy[r] = ngz[r] = 0.0f;
-            if (next_in && x_in && y_in[r]) {
-                const uint64_t i = kn + column(r);
-                nex[r] = e_in[i];
-                if (r < R - 1) ney[r] = e_in[cells + i];  // row R−1 only feeds Hx/Hz of row R−2
-                nez[r] = e_in[2 * cells + i];
-            }
-            if (next_in && xh_in && y_in[r]) {
-                const uint64_t i = kn + halo_column(r);
-                ngx[r] = e_in[i];
-                if (r < R - 1) {
-                    ngy[r] = e_in[cells + i];
-                    ngz[r] = e_in[2 * cells + i];
-                }
-            }
-            if (r < R - 1) {
-                hx[r] = hy[r] = hz[r] = hgy[r] = hgz[r] = 0.0f;
-                mat[r] = hmat[r] = 0;
-                if (x_in && y_in[r]) {
-                    const uint64_t i = kp + column(r);
-                    hx[r] = h_in[i];
-                    if (r > 0) hy[r] = h_in[cells + i];  // row 0 supplies only Hx and Hz
-                    hz[r] = h_in[2 * cells + i];
-                    mat[r] = material[i];
-                }
-                if (r > 0 && lane == 0 && xh_in && y_in[r]) {
-                    const uint64_t i = kp + halo_column(r);
-                    hgy[r] = h_in[cells + i];
-                    hgz[r] = h_in[2 * cells + i];
-                    hmat[r] = material[i];
-                }
-            }
-        }
-
-        // H_new(k), owned lanes plus lane 0's x0−1 halo (Hy, Hz).
-        float hy_lo[R - 1], hz_lo[R - 1];
-#pragma unroll
-        for (int r = 0; r < R - 1; ++r) {
-            // The shuffles run on every lane before the select: a cross-lane
-            // read from a lane that is switched off by a branch is undefined.
-            const float ez_down = __shfl_down(ez[r], 1, kLanes);
-            const float ey_down = __shfl_down(ey[r], 1, kLanes);
-            const float ez_xp = lane == kLanes - 1 ? gz[r] : ez_down;
-            const float ey_xp = lane == kLanes - 1 ? gy[r] : ey_down;
-            const float4 c = s_coeff[mat[r]];
-            const float mg = c.z, ml = c.w;
-            {
-                const float curl = gain_h_y[r] * (ez[r + 1] - ez[r]) - gain_h_z * (ney[r] - ey[r]);
-                const float loss = (ml + (loss_half_y[r] + loss_half_z)) + loss_int_x;
-                hx[r] = magnetic_update<1>(hx[r], loss, mg, curl);
-            }
-            if (r > 0) {
-                const float curl = gain_h_z * (nex[r] - ex[r]) - gain_h_x * (ez_xp - ez[r]);
-                const float loss = (ml + (loss_int_y[r] + loss_half_z)) + loss_half_x;
-                hy[r] = magnetic_update<1>(hy[r], loss, mg, curl);
-            }
-            {
-                const float curl = gain_h_x * (ey_xp - ey[r]) - gain_h_y[r] * (ex[r + 1] - ex[r]);
-                const float loss = (ml + (loss_half_y[r] + loss_int_z)) + loss_half_x;
-                hz[r] = magnetic_update<1>(hz[r], loss, mg, curl);
-            }
-            hy_lo[r] = hz_lo[r] = 0.0f;
-            if (r > 0) {
-                // Lane 0: the cell at x0−1, whose x+1 neighbour is lane 0's own.
-                const float4 cl = s_coeff[hmat[r]];
-                const float mgl = cl.z, mll = cl.w;
-                {
-                    const float curl = gain_h_z * (ngx[r] - gx[r]) - gain_h_xl * (ez[r] - gz[r]);
-                    const float loss = (mll + (loss_int_y[r] + loss_half_z)) + loss_half_xl;
-                    hy_lo[r] = magnetic_update<1>(hgy[r], loss, mgl, curl);
-                }
-                {
-                    const float curl = gain_h_xl * (ey[r] - gy[r]) - gain_h_y[r] * (gx[r + 1] - gx[r]);
-                    const float loss = (mll + (loss_half_y[r] + loss_int_z)) + loss_half_xl;
-                    hz_lo[r] = magnetic_update<1>(hgz[r], loss, mgl, curl);
-                }
-            }
-            if (p.n_magnetic) {
-                const uint32_t yy = uint32_t(y0 + r), xx = uint32_t(x);
-                hx[r] = inject(hx[r], 0, xx, yy, k, magnetic_list, p.n_magnetic, weights);
-                hy[r] = inject(hy[r], 1, xx, yy, k, magnetic_list, p.n_magnetic, weights);
-                hz[r] = inject(hz[r], 2, xx, yy, k, magnetic_list, p.n_magnetic, weights);
-                if (r > 0) {
-                    const uint32_t xl = uint32_t(x - 1);
-                    hy_lo[r] = inject(hy_lo[r], 1, xl, yy, k, magnetic_list, p.n_magnetic, weights);
-                    hz_lo[r] = inject(hz_lo[r], 2, xl, yy, k, magnetic_list, p.n_magnetic, weights);
-                }
-            }
-        }
-
-        if (k >= zb) {
-#pragma unroll
-            for (int r = 1; r < R - 1; ++r) {
-                const float hy_up = __shfl_up(hy[r], 1, kLanes);
-                const float hz_up = __shfl_up(hz[r], 1, kLanes);
-                const float hy_xm = lane == 0 ? hy_lo[r] : hy_up;
-                const float hz_xm = lane == 0 ? hz_lo[r] : hz_up;
-                const int y = y0 + r;
-                const float4 c = s_coeff[mat[r]];
-                const float eg = c.x, el = c.y;
-                float ex_new = ex[r], ey_new = ey[r], ez_new = ez[r];
-                if (y > 0 && k > 0) {
-                    const float curl = gain_e_y[r] * (hz[r] - hz[r - 1]) - gain_e_z * (hy[r] - hy_prev[r]);
-                    const float loss = (el + (loss_int_y[r] + loss_int_z)) + loss_half_x;
-                    ex_new = electric_update<1>(ex[r], loss, eg, curl);
-                }
-                if (k > 0 && x > 0) {
-                    const float curl = gain_e_z * (hx[r] - hx_prev[r]) - gain_e_x * (hz[r] - hz_xm);
-                    const float loss = (el + (loss_half_y[r] + loss_int_z)) + loss_int_x;
-                    ey_new = electric_update<1>(ey[r], loss, eg, curl);
-                }
-                if (x > 0 && y > 0) {
-                    const float curl = gain_e_x * (hy[r] - hy_xm) - gain_e_y[r] * (hx[r] - hx[r - 1]);
-                    const float loss = (el + (loss_int_y[r] + loss_half_z)) + loss_int_x;
-                    ez_new = electric_update<1>(ez[r], loss, eg, curl);
-                }
-                if (p.n_electric) {
-                    const uint32_t yy = uint32_t(y), xx = uint32_t(x);
-                    ex_new = inject(ex_new, 0, xx, yy, k, electric_list, p.n_electric, weights);
-                    ey_new = inject(ey_new, 1, xx, yy, k, electric_list, p.n_electric, weights);
-                    ez_new = inject(ez_new, 2, xx, yy, k, electric_list, p.n_electric, weights);
-                }
-                if (x_in && y_in[r]) {
-                    const uint64_t i = kp + column(r);
-                    store<kNonTemporal>(h_out + i, hx[r]);
-                    store<kNonTemporal>(h_out + cells + i, hy[r]);
-                    store<kNonTemporal>(h_out + 2 * cells + i, hz[r]);
-                    store<kNonTemporal>(e_out + i, ex_new);
-                    store<kNonTemporal>(e_out + cells + i, ey_new);
-                    store<kNonTemporal>(e_out + 2 * cells + i, ez_new);
-                    if (p.intensity) p.intensity[i] += double(intensity_term(ex_new, ey_new, ez_new));
-                }
-            }
-        }
-#pragma unroll
-        for (int r = 0; r < R - 1; ++r) {
-            hx_prev[r] = hx[r];
-            hy_prev[r] = hy[r];
-        }
-#pragma unroll
-        for (int r = 0; r < R; ++r) {
-            ex[r] = nex[r];
-            ey[r] = ney[r];
-            ez[r] = nez[r];
-            gx[r] = ngx[r];
-            gy[r] = ngy[r];
-            gz[r] = ngz[r];
-        }
-    }
-}
+#include "fused_aligned.inc"
 
 // ---------------------------------------------------------------------------
 // Two-pass path: the reference's dispatch shape (src/shaders/fdtd.wgsl:139-214),
@@ -772,14 +493,15 @@ void launch_fused(const StepArgs &args, int rows, hipStream_t stream) {
     }
 }
 
-void launch_fused_aligned(const StepArgs &args, int rows, bool nontemporal, hipStream_t stream) {
+template <typename Offset>
+static void launch_fused_aligned_as(const StepArgs &args, int rows, bool nontemporal, hipStream_t stream) {
     const uint32_t blocks = (args.tiles_total + 3) / 4;
 #define DPA_ROWS(R)                                                                \
     case R:                                                                        \
         if (nontemporal)                                                           \
-            k_fused_aligned<R, true><<<blocks, 256, 0, stream>>>(args);            \
+            k_fused_aligned<R, true, Offset><<<blocks, 256, 0, stream>>>(args);    \
         else                                                                       \
-            k_fused_aligned<R, false><<<blocks, 256, 0, stream>>>(args);           \
+            k_fused_aligned<R, false, Offset><<<blocks, 256, 0, stream>>>(args);   \
         break;
     switch (rows) {
         DPA_ROWS(4)
@@ -788,10 +510,18 @@ void launch_fused_aligned(const StepArgs &args, int rows, bool nontemporal, hipS
         DPA_ROWS(10)
         DPA_ROWS(12)
     default:
-        k_fused_aligned<6, false><<<blocks, 256, 0, stream>>>(args);
+        k_fused_aligned<4, true, Offset><<<blocks, 256, 0, stream>>>(args);
         break;
     }
 #undef DPA_ROWS
+}
+
+void launch_fused_aligned(const StepArgs &args, int rows, bool nontemporal, hipStream_t stream) {
+    // 32-bit byte offsets while one component fits in 4 GiB (1 Gi cells).
+    if (args.cells * sizeof(float) <= 0xFFFFFFFFull)
+        launch_fused_aligned_as<uint32_t>(args, rows, nontemporal, stream);
+    else
+        launch_fused_aligned_as<uint64_t>(args, rows, nontemporal, stream);
 }
 
 void launch_update_h(const StepArgs &args, int direction, hipStream_t stream) {
