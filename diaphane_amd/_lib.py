@@ -77,6 +77,21 @@ def _declare(lib: ctypes.CDLL) -> None:
         "dp_sim_elapsed_ms": (c_int, [sim, c_int, c_int, POINTER(c_float)]),
         "dp_sim_launch_count": (c_uint64, [sim]),
         "dp_sim_stream_probe": (c_int, [sim, c_int, POINTER(c_double)]),
+        "dp_batch_create": (c_int, [c_void_p, POINTER(c_void_p)]),
+        "dp_batch_destroy": (None, [c_void_p]),
+        "dp_batch_add": (c_int, [c_void_p, sim]),
+        "dp_batch_remove": (c_int, [c_void_p, sim]),
+        "dp_batch_set_target": (c_int, [c_void_p, sim, c_uint64]),
+        "dp_batch_advance": (c_int, [c_void_p, c_uint64]),
+        "dp_batch_wait": (c_int, [c_void_p, c_uint32]),
+        "dp_batch_set_graph_steps": (c_int, [c_void_p, c_uint32]),
+        "dp_batch_set_tile": (c_int, [c_void_p, c_int, c_int, c_int]),
+        "dp_batch_autotune": (c_int, [c_void_p, c_uint32, POINTER(c_int), POINTER(c_int), POINTER(c_float)]),
+        "dp_batch_size": (c_uint32, [c_void_p]),
+        "dp_batch_cells": (c_uint64, [c_void_p]),
+        "dp_batch_launch_count": (c_uint64, [c_void_p]),
+        "dp_batch_record_event": (c_int, [c_void_p, c_int]),
+        "dp_batch_elapsed_ms": (c_int, [c_void_p, c_int, c_int, POINTER(c_float)]),
     }
     for name, (restype, argtypes) in table.items():
         fn = getattr(lib, name)

@@ -36,6 +36,7 @@ SOURCES = [
     ("kernels.hip", DEVICE),
     ("scene.cpp", HOST),
     ("engine.cpp", HOST),
+    ("batch.cpp", HOST),
 ]
 
 

@@ -1,0 +1,474 @@
+// The batched form: many independent sims of uneven sizes advanced together
+// by one persistent kernel launch per step over a packed instance table.
+//
+// A batch borrows ordinary dp_sims. While a sim is a member, the batch
+// advances it (its own advance entry point refuses) and its results are read
+// through the ordinary dp_sim_read_* calls. Each member may carry a target
+// step; a member at its target sits out the remaining launches, which is
+// what lets a server admit and retire requests between chunks.
+//
+// A chunk of K steps is prepared on the host — per step and member a
+// StepSlot (active, ping-pong parity, injections) and the injection list —
+// uploaded in one copy, then launched as K kernel nodes of a captured HIP
+// graph (re-captured only when the membership or K changes). Two staging sets
+// alternate so the host prepares chunk n+1 while the device runs chunk n.
+#include "engine.hpp"
+
+#include <limits>
+
+namespace {
+
+struct StagingSet {
+    PinnedBuffer host;        // slots then injections
+    DeviceBuffer slots;       // K × members StepSlot
+    DeviceBuffer injections;  // InjDesc list of the chunk
+    hipEvent_t done = nullptr;
+    bool in_flight = false;
+    hipGraphExec_t graph = nullptr;
+    uint32_t graph_steps = 0;
+    uint64_t graph_version = 0;
+
+    void drop_graph() {
+        if (graph) (void)hipGraphExecDestroy(graph);
+        graph = nullptr;
+        graph_steps = 0;
+    }
+};
+
+}  // namespace
+
+struct dp_batch {
+    dp_engine *engine = nullptr;
+    hipStream_t stream = nullptr;
+    std::vector<dp_sim *> members;
+    std::vector<uint64_t> targets;
+    bool dirty = true;
+    uint64_t version = 0;
+
+    int rows = 4;
+    uint32_t depth = 32;
+    int flags = DP_TILE_NONTEMPORAL;
+    uint32_t graph_steps = 32;
+
+    DeviceBuffer instance_table;
+    DeviceBuffer owners;
+    std::vector<const void *> table_intensity;  // per member, as the table holds it
+    DeviceBuffer trial_slots, trial_injections;
+    uint32_t n_tiles = 0;
+    uint32_t blocks = 0;
+    StagingSet sets[2];
+    int next_set = 0;
+    hipEvent_t events[8] = {};
+    uint64_t launches = 0;
+
+    ~dp_batch() {
+        if (stream) (void)hipStreamSynchronize(stream);
+        for (dp_sim *sim : members) restore(sim);
+        for (auto &s : sets) {
+            s.drop_graph();
+            if (s.done) (void)hipEventDestroy(s.done);
+        }
+        for (auto &e : events)
+            if (e) (void)hipEventDestroy(e);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+
+    static void restore(dp_sim *sim) {
+        sim->stream = sim->own_stream;
+        sim->own_stream = nullptr;
+        sim->batch = nullptr;
+    }
+
+    void detach(dp_sim *sim) {
+        const size_t i = index_of(sim);
+        if (i == members.size()) return;
+        sync();
+        for (auto &set : sets) set.in_flight = false;
+        members.erase(members.begin() + static_cast<std::ptrdiff_t>(i));
+        targets.erase(targets.begin() + static_cast<std::ptrdiff_t>(i));
+        restore(sim);
+        dirty = true;
+    }
+
+    size_t index_of(const dp_sim *sim) const {
+        for (size_t i = 0; i < members.size(); ++i)
+            if (members[i] == sim) return i;
+        return members.size();
+    }
+
+    void sync() { check(hipStreamSynchronize(stream), "hipStreamSynchronize"); }
+
+    void tiles_of(const dp_sim *sim, InstanceDesc &d) const {
+        const uint32_t nx = sim->grid.extent[0], ny = sim->grid.extent[1], nz = sim->grid.extent[2];
+        d.tiles_x = (nx + kAlignedX - 1) / kAlignedX;
+        d.tiles_y = (ny + (rows - 2) - 1) / (rows - 2);
+        d.depth = std::min<uint32_t>(nz, depth);
+        d.tiles_z = (nz + d.depth - 1) / d.depth;
+        d.tiles_total = d.tiles_x * d.tiles_y * d.tiles_z;
+    }
+
+    // Instance table and tile owners for the current membership.
+    void rebuild() {
+        sync();
+        std::vector<InstanceDesc> table(members.size());
+        std::vector<uint32_t> owner;
+        table_intensity.assign(members.size(), nullptr);
+        for (dp_sim *sim : members)
+            if (sim->accumulating) sim->ensure_intensity();
+        for (size_t i = 0; i < members.size(); ++i) {
+            dp_sim *s = members[i];
+            InstanceDesc &d = table[i];
+            d = InstanceDesc{};
+            d.e[0] = s->e[0].as<float>();
+            d.e[1] = s->e[1].as<float>();
+            d.h[0] = s->h[0].as<float>();
+            d.h[1] = s->h[1].as<float>();
+            d.material = s->material.as<uint8_t>();
+            d.coefficients = s->coefficients.as<float4>();
+            d.axis_table = s->axis_table.as<float>();
+            d.weights = s->weights.as<float>();
+            d.intensity = s->intensity.as<double>();
+            d.nx = s->grid.extent[0];
+            d.ny = s->grid.extent[1];
+            d.nz = s->grid.extent[2];
+            d.material_count = s->material_count;
+            d.cells = s->cells;
+            tiles_of(s, d);
+            d.tile_base = static_cast<uint32_t>(owner.size());
+            table_intensity[i] = s->intensity.ptr;
+            owner.insert(owner.end(), d.tiles_total, static_cast<uint32_t>(i));
+        }
+        if (owner.size() > std::numeric_limits<uint32_t>::max() / 2)
+            fail(DP_ERR_UNSUPPORTED, "too many tiles in one batch");
+        n_tiles = static_cast<uint32_t>(owner.size());
+        instance_table.allocate(std::max<size_t>(table.size(), 1) * sizeof(InstanceDesc));
+        owners.allocate(std::max<size_t>(owner.size(), 1) * sizeof(uint32_t));
+        if (!table.empty())
+            check(hipMemcpy(instance_table.ptr, table.data(), table.size() * sizeof(InstanceDesc),
+                            hipMemcpyHostToDevice),
+                  "hipMemcpy");
+        if (!owner.empty())
+            check(hipMemcpy(owners.ptr, owner.data(), owner.size() * sizeof(uint32_t), hipMemcpyHostToDevice),
+                  "hipMemcpy");
+        const uint32_t resident = batch_resident_blocks(rows, (flags & DP_TILE_NONTEMPORAL) != 0);
+        blocks = std::max<uint32_t>(1, std::min<uint32_t>(resident, (n_tiles + 3) / 4));
+        ++version;
+        dirty = false;
+    }
+
+    // Times `trials` launches of the current tiling without changing any
+    // member: every member reads its current buffers and writes the other
+    // half of its ping-pong pair (which the next real step overwrites), with
+    // no injections and no intensity.
+    float time_tiling(uint32_t trials) {
+        if (dirty) rebuild();
+        std::vector<StepSlot> slot(members.size());
+        for (size_t i = 0; i < members.size(); ++i) {
+            slot[i] = StepSlot{};
+            slot[i].active = 1;
+            slot[i].parity = static_cast<uint32_t>(members[i]->current);
+        }
+        trial_slots.allocate(std::max<size_t>(slot.size(), 1) * sizeof(StepSlot));
+        check(hipMemcpy(trial_slots.ptr, slot.data(), slot.size() * sizeof(StepSlot), hipMemcpyHostToDevice),
+              "hipMemcpy");
+        if (!trial_injections.ptr) trial_injections.allocate(sizeof(InjDesc));
+        const bool nt = (flags & DP_TILE_NONTEMPORAL) != 0;
+        auto launch = [&] {
+            launch_fused_batch(instance_table.as<InstanceDesc>(), trial_slots.as<StepSlot>(), owners.as<uint32_t>(),
+                               n_tiles, trial_injections.as<InjDesc>(), rows, nt, blocks, stream);
+        };
+        launch();  // first launch of a shape loads its code object
+        check(hipGetLastError(), "kernel launch");
+        check(hipEventRecord(events[6], stream), "hipEventRecord");
+        for (uint32_t t = 0; t < trials; ++t) launch();
+        check(hipEventRecord(events[7], stream), "hipEventRecord");
+        check(hipEventSynchronize(events[7]), "hipEventSynchronize");
+        float ms = 0.0f;
+        check(hipEventElapsedTime(&ms, events[6], events[7]), "hipEventElapsedTime");
+        return ms / static_cast<float>(trials);
+    }
+
+    // Picks rows and slab depth by timing every candidate on the members as
+    // they are. Results never depend on the choice.
+    void autotune(uint32_t trials, int *rows_out, int *depth_out, float *ms_out) {
+        static const int row_choices[] = {4, 6, 8};
+        static const uint32_t depth_choices[] = {8, 16, 32, 64};
+        int best_rows = rows;
+        uint32_t best_depth = depth;
+        float best = std::numeric_limits<float>::infinity();
+        if (!members.empty()) {
+            for (int r : row_choices)
+                for (uint32_t d : depth_choices) {
+                    rows = r;
+                    depth = d;
+                    dirty = true;
+                    const float ms = time_tiling(trials);
+                    if (ms < best) {
+                        best = ms;
+                        best_rows = r;
+                        best_depth = d;
+                    }
+                }
+        }
+        rows = best_rows;
+        depth = best_depth;
+        dirty = true;
+        for (auto &set : sets) set.drop_graph();
+        if (rows_out) *rows_out = rows;
+        if (depth_out) *depth_out = static_cast<int>(depth);
+        if (ms_out) *ms_out = members.empty() ? 0.0f : best;
+    }
+
+    void launch_chunk(StagingSet &set, uint32_t steps) {
+        const StepSlot *slots = set.slots.as<StepSlot>();
+        const InjDesc *inj = set.injections.as<InjDesc>();
+        const bool nt = (flags & DP_TILE_NONTEMPORAL) != 0;
+        auto launch_all = [&] {
+            for (uint32_t s = 0; s < steps; ++s)
+                launch_fused_batch(instance_table.as<InstanceDesc>(), slots + size_t(s) * members.size(),
+                                   owners.as<uint32_t>(), n_tiles, inj, rows, nt, blocks, stream);
+        };
+        if (graph_steps == 0 || steps != graph_steps) {
+            launch_all();
+            check(hipGetLastError(), "kernel launch");
+            return;
+        }
+        if (!set.graph || set.graph_version != version || set.graph_steps != steps) {
+            set.drop_graph();
+            hipGraph_t graph = nullptr;
+            check(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+            launch_all();
+            check(hipStreamEndCapture(stream, &graph), "hipStreamEndCapture");
+            const hipError_t made = hipGraphInstantiate(&set.graph, graph, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(graph);
+            check(made, "hipGraphInstantiate");
+            set.graph_steps = steps;
+            set.graph_version = version;
+        }
+        check(hipGraphLaunch(set.graph, stream), "hipGraphLaunch");
+    }
+
+    // Advances every member by min(steps, target − step).
+    void advance(uint64_t steps) {
+        if (members.empty() || steps == 0) return;
+        // A member that switched intensity on since the table was built
+        // brings a new buffer: the table must carry it.
+        for (size_t i = 0; i < members.size() && !dirty; ++i) {
+            if (members[i]->accumulating) members[i]->ensure_intensity();
+            dirty = members[i]->intensity.ptr != table_intensity[i];
+        }
+        if (dirty) rebuild();
+        const size_t n = members.size();
+        std::vector<StepSlot> slot_buf;
+        std::vector<InjDesc> inj_all, inj_one;
+        while (steps > 0) {
+            bool any = false;
+            for (size_t i = 0; i < n; ++i) any = any || members[i]->step < targets[i];
+            if (!any) return;
+            const uint32_t chunk = static_cast<uint32_t>(
+                std::min<uint64_t>(steps, graph_steps ? graph_steps : 64));
+            slot_buf.assign(size_t(chunk) * n, StepSlot{});
+            inj_all.clear();
+            for (size_t i = 0; i < n; ++i) {
+                dp_sim *sim = members[i];
+                const uint64_t room = targets[i] > sim->step ? targets[i] - sim->step : 0;
+                const uint32_t active = static_cast<uint32_t>(std::min<uint64_t>(room, chunk));
+                if (active == 0) continue;
+                auto table = sim->build_injections(sim->step, active, +1, inj_one);
+                int parity = sim->current;
+                for (uint32_t s = 0; s < active; ++s) {
+                    StepSlot &slot = slot_buf[size_t(s) * n + i];
+                    slot.active = 1;
+                    slot.parity = static_cast<uint32_t>(parity);
+                    slot.inj_offset = static_cast<uint32_t>(inj_all.size()) + table[s].offset;
+                    slot.n_magnetic = table[s].magnetic;
+                    slot.n_electric = table[s].electric;
+                    slot.accumulate = sim->accumulating ? 1u : 0u;
+                    parity = 1 - parity;
+                }
+                inj_all.insert(inj_all.end(), inj_one.begin(), inj_one.end());
+            }
+            StagingSet &set = sets[next_set];
+            next_set ^= 1;
+            if (set.in_flight) {
+                check(hipEventSynchronize(set.done), "hipEventSynchronize");
+                set.in_flight = false;
+            }
+            const size_t slot_bytes = slot_buf.size() * sizeof(StepSlot);
+            const size_t inj_bytes = std::max<size_t>(inj_all.size(), 1) * sizeof(InjDesc);
+            set.host.reserve(slot_bytes + inj_bytes);
+            std::memcpy(set.host.ptr, slot_buf.data(), slot_bytes);
+            if (!inj_all.empty())
+                std::memcpy(static_cast<char *>(set.host.ptr) + slot_bytes, inj_all.data(),
+                            inj_all.size() * sizeof(InjDesc));
+            if (set.slots.bytes < slot_bytes) {
+                set.slots.allocate(slot_bytes);
+                set.drop_graph();
+            }
+            if (set.injections.bytes < inj_bytes) {
+                set.injections.allocate(inj_bytes * 2);
+                set.drop_graph();
+            }
+            check(hipMemcpyAsync(set.slots.ptr, set.host.ptr, slot_bytes, hipMemcpyHostToDevice, stream),
+                  "hipMemcpyAsync");
+            check(hipMemcpyAsync(set.injections.ptr, static_cast<char *>(set.host.ptr) + slot_bytes, inj_bytes,
+                                 hipMemcpyHostToDevice, stream),
+                  "hipMemcpyAsync");
+            launch_chunk(set, chunk);
+            check(hipEventRecord(set.done, stream), "hipEventRecord");
+            set.in_flight = true;
+            launches += chunk;
+            for (size_t i = 0; i < n; ++i) {
+                dp_sim *sim = members[i];
+                const uint64_t room = targets[i] > sim->step ? targets[i] - sim->step : 0;
+                const uint32_t active = static_cast<uint32_t>(std::min<uint64_t>(room, chunk));
+                sim->step += active;
+                if (active & 1) sim->current = 1 - sim->current;
+                if (sim->accumulating) sim->accumulated += active;
+                sim->launches += active;
+            }
+            steps -= chunk;
+        }
+    }
+
+};
+
+extern "C" {
+
+int dp_batch_create(dp_engine *engine, dp_batch **out) {
+    return guarded([&] {
+        if (!engine || !out) fail(DP_ERR_INVALID, "null argument");
+        check(hipSetDevice(engine->device), "hipSetDevice");
+        auto batch = std::make_unique<dp_batch>();
+        batch->engine = engine;
+        check(hipStreamCreateWithFlags(&batch->stream, hipStreamNonBlocking), "hipStreamCreate");
+        for (auto &s : batch->sets) check(hipEventCreateWithFlags(&s.done, hipEventDisableTiming), "hipEventCreate");
+        for (auto &e : batch->events) check(hipEventCreate(&e), "hipEventCreate");
+        *out = batch.release();
+    });
+}
+
+void dp_batch_destroy(dp_batch *batch) { delete batch; }
+
+int dp_batch_add(dp_batch *batch, dp_sim *sim) {
+    return guarded([&] {
+        if (!batch || !sim) fail(DP_ERR_INVALID, "null argument");
+        if (sim->batch) fail(DP_ERR_INVALID, "the sim already belongs to a batch");
+        if (sim->cells * sizeof(float) > 0xFFFFFFFFull)
+            fail(DP_ERR_UNSUPPORTED, "a batched grid must fit 4 GiB per field component");
+        sim->sync();
+        batch->sync();
+        sim->batch = batch;
+        sim->own_stream = sim->stream;
+        sim->stream = batch->stream;
+        batch->members.push_back(sim);
+        batch->targets.push_back(std::numeric_limits<uint64_t>::max());
+        batch->dirty = true;
+    });
+}
+
+int dp_batch_remove(dp_batch *batch, dp_sim *sim) {
+    return guarded([&] {
+        if (!batch || !sim) fail(DP_ERR_INVALID, "null argument");
+        const size_t i = batch->index_of(sim);
+        if (i == batch->members.size()) fail(DP_ERR_INVALID, "the sim is not a member of this batch");
+        batch->detach(sim);
+    });
+}
+
+int dp_batch_set_target(dp_batch *batch, dp_sim *sim, uint64_t step) {
+    return guarded([&] {
+        if (!batch || !sim) fail(DP_ERR_INVALID, "null argument");
+        const size_t i = batch->index_of(sim);
+        if (i == batch->members.size()) fail(DP_ERR_INVALID, "the sim is not a member of this batch");
+        batch->targets[i] = step;
+    });
+}
+
+int dp_batch_advance(dp_batch *batch, uint64_t steps) {
+    return guarded([&] {
+        if (!batch) fail(DP_ERR_INVALID, "null batch");
+        batch->advance(steps);
+    });
+}
+
+int dp_batch_wait(dp_batch *batch, uint32_t timeout_ms) {
+    return guarded([&] {
+        if (!batch) fail(DP_ERR_INVALID, "null batch");
+        const uint32_t limit = timeout_ms ? timeout_ms : 120000;
+        const auto start = std::chrono::steady_clock::now();
+        for (;;) {
+            hipError_t status = hipStreamQuery(batch->stream);
+            if (status == hipSuccess) return;
+            if (status != hipErrorNotReady) check(status, "hipStreamQuery");
+            if (std::chrono::steady_clock::now() - start > std::chrono::milliseconds(limit))
+                fail(DP_ERR_TIMEOUT, "the GPU did not finish in time");
+            std::this_thread::sleep_for(std::chrono::microseconds(100));
+        }
+    });
+}
+
+int dp_batch_set_graph_steps(dp_batch *batch, uint32_t steps) {
+    return guarded([&] {
+        if (!batch) fail(DP_ERR_INVALID, "null batch");
+        if (steps > 4096) fail(DP_ERR_INVALID, "at most 4096 steps per graph");
+        batch->graph_steps = steps;
+    });
+}
+
+int dp_batch_autotune(dp_batch *batch, uint32_t trials, int *rows, int *depth, float *ms_per_step) {
+    return guarded([&] {
+        if (!batch) fail(DP_ERR_INVALID, "null batch");
+        if (trials == 0 || trials > 1000) fail(DP_ERR_INVALID, "trials must be in 1..1000");
+        batch->sync();
+        for (auto &set : batch->sets) set.in_flight = false;
+        batch->autotune(trials, rows, depth, ms_per_step);
+    });
+}
+
+int dp_batch_set_tile(dp_batch *batch, int rows, int depth, int flags) {
+    return guarded([&] {
+        if (!batch) fail(DP_ERR_INVALID, "null batch");
+        if (rows != 0 && rows != 4 && rows != 6 && rows != 8) fail(DP_ERR_INVALID, "rows must be 4, 6 or 8");
+        if (depth < 0) fail(DP_ERR_INVALID, "depth must be non-negative");
+        batch->sync();
+        batch->rows = rows ? rows : 4;
+        batch->depth = depth ? static_cast<uint32_t>(depth) : 32;
+        if (flags >= 0) batch->flags = flags & DP_TILE_NONTEMPORAL;
+        batch->dirty = true;
+    });
+}
+
+uint32_t dp_batch_size(const dp_batch *batch) { return batch ? static_cast<uint32_t>(batch->members.size()) : 0; }
+
+uint64_t dp_batch_cells(const dp_batch *batch) {
+    uint64_t total = 0;
+    if (batch)
+        for (const dp_sim *s : batch->members) total += s->cells;
+    return total;
+}
+
+uint64_t dp_batch_launch_count(const dp_batch *batch) { return batch ? batch->launches : 0; }
+
+int dp_batch_record_event(dp_batch *batch, int slot) {
+    return guarded([&] {
+        if (!batch || slot < 0 || slot >= 8) fail(DP_ERR_INVALID, "bad event slot");
+        check(hipEventRecord(batch->events[slot], batch->stream), "hipEventRecord");
+    });
+}
+
+int dp_batch_elapsed_ms(dp_batch *batch, int from_slot, int to_slot, float *ms) {
+    return guarded([&] {
+        if (!batch || !ms || from_slot < 0 || from_slot >= 8 || to_slot < 0 || to_slot >= 8)
+            fail(DP_ERR_INVALID, "bad event slot");
+        check(hipEventSynchronize(batch->events[to_slot]), "hipEventSynchronize");
+        check(hipEventElapsedTime(ms, batch->events[from_slot], batch->events[to_slot]), "hipEventElapsedTime");
+    });
+}
+
+}  // extern "C"
+
+dp_sim::~dp_sim() {
+    if (batch) batch->detach(this);
+    release_stream();
+}

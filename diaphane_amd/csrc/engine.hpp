@@ -1,0 +1,321 @@
+// Device engine internals shared by the single-sim and batch entry points:
+// the sim object (gpu::Simulation's counterpart, src/gpu.rs:159-799) and
+// the RAII helpers around HIP allocations.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "kernels.hpp"
+#include "scene.hpp"
+
+using namespace dpa;
+
+
+namespace dpa_detail {
+
+struct HipFailure {
+    hipError_t code;
+    const char *what;
+};
+
+inline void check(hipError_t code, const char *what) {
+    if (code != hipSuccess) throw HipFailure{code, what};
+}
+
+template <typename F>
+inline int guarded(F &&body) {
+    try {
+        body();
+        return DP_OK;
+    } catch (const Error &e) {
+        set_error(e.message);
+        return e.code;
+    } catch (const HipFailure &f) {
+        set_error(std::string(f.what) + ": " + hipGetErrorString(f.code));
+        return DP_ERR_DEVICE;
+    } catch (const std::bad_alloc &) {
+        set_error("host allocation failed");
+        return DP_ERR_DEVICE;
+    }
+}
+
+[[noreturn]] inline void fail(int code, const std::string &message) { throw Error{code, message}; }
+
+// Device allocation owned by a sim.
+struct DeviceBuffer {
+    void *ptr = nullptr;
+    size_t bytes = 0;
+    DeviceBuffer() = default;
+    DeviceBuffer(const DeviceBuffer &) = delete;
+    DeviceBuffer &operator=(const DeviceBuffer &) = delete;
+    ~DeviceBuffer() { release(); }
+    void allocate(size_t n) {
+        release();
+        if (n == 0) n = 16;
+        check(hipMalloc(&ptr, n), "hipMalloc");
+        bytes = n;
+    }
+    void release() {
+        if (ptr) (void)hipFree(ptr);
+        ptr = nullptr;
+        bytes = 0;
+    }
+    template <typename T>
+    T *as() const {
+        return static_cast<T *>(ptr);
+    }
+};
+
+// Pinned host staging, grown on demand.
+struct PinnedBuffer {
+    void *ptr = nullptr;
+    size_t bytes = 0;
+    ~PinnedBuffer() {
+        if (ptr) (void)hipHostFree(ptr);
+    }
+    void reserve(size_t n) {
+        if (n <= bytes) return;
+        if (ptr) (void)hipHostFree(ptr);
+        ptr = nullptr;
+        check(hipHostMalloc(&ptr, n, hipHostMallocDefault), "hipHostMalloc");
+        bytes = n;
+    }
+};
+
+}  // namespace dpa_detail
+
+using namespace dpa_detail;
+
+struct dp_batch;
+
+struct dp_engine {
+    int device = 0;
+    std::string arch;
+};
+
+struct dp_sim {
+    dp_engine *engine = nullptr;
+    Grid grid;
+    float time_step = 0.0f;
+    std::vector<dp_source> sources;
+    std::vector<int32_t> source_weight_offset;  // per source, −1 if unapodized
+    uint32_t material_count = 0;
+    bool reversible = false;
+    uint64_t cells = 0;
+
+    hipStream_t stream = nullptr;
+    DeviceBuffer e[2], h[2];
+    DeviceBuffer material, coefficients, axis_table, weights, eps_mu, widths;
+    DeviceBuffer intensity, injections, partials;
+    PinnedBuffer staging;
+    int current = 0;  // which of the ping-pong pair holds the state
+
+    uint64_t step = 0;
+    bool accumulating = false;
+    uint64_t accumulated = 0;
+    int kernel = DP_KERNEL_FUSED;
+    int rows = 0;  // 0: tuned default
+    uint32_t depth = 0;  // 0: tuned default
+    // Measured on MI355X at 768³: non-temporal stores help (~2%); renumbering
+    // workgroups per XCD does not (the halos already hit L2).
+    int tile_flags = DP_TILE_NONTEMPORAL;
+    uint64_t launches = 0;
+    hipEvent_t events[8] = {};
+
+    ~dp_sim();  // leaves its batch first (batch.cpp)
+
+    void release_stream() {
+        if (stream) (void)hipStreamSynchronize(stream);
+        for (auto &ev : events)
+            if (ev) (void)hipEventDestroy(ev);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+
+    // Set while a batch advances this sim. A member's `stream` is the batch's
+    // stream, so its own copies, clears and waits are ordered with the batch
+    // launches; `own_stream` keeps the sim's stream until it leaves.
+    dp_batch *batch = nullptr;
+    hipStream_t own_stream = nullptr;
+
+    void sync() { check(hipStreamSynchronize(stream), "hipStreamSynchronize"); }
+
+    StepArgs args(int from, int to) const {
+        StepArgs a{};
+        a.e_in = e[from].as<float>();
+        a.h_in = h[from].as<float>();
+        a.e_out = e[to].as<float>();
+        a.h_out = h[to].as<float>();
+        a.material = material.as<uint8_t>();
+        a.coefficients = coefficients.as<float4>();
+        a.axis_table = axis_table.as<float>();
+        a.weights = weights.as<float>();
+        a.material_count = material_count;
+        a.nx = grid.extent[0];
+        a.ny = grid.extent[1];
+        a.nz = grid.extent[2];
+        a.cells = cells;
+        return a;
+    }
+
+    int effective_rows() const {
+        if (rows > 0) return rows;
+        // Measured on MI355X at 768³: 4 rows (96 VGPRs) beats 6 (141) and 8 (178)
+        // for the aligned kernel; the overlap kernel peaks at 6.
+        return kernel == DP_KERNEL_FUSED_OVERLAP ? 6 : 4;
+    }
+
+    void ensure_intensity() {
+        if (!intensity.ptr) {
+            intensity.allocate(cells * sizeof(double));
+            check(hipMemsetAsync(intensity.ptr, 0, cells * sizeof(double), stream), "hipMemsetAsync");
+        }
+    }
+
+    void clear_intensity() {
+        if (accumulated == 0) return;
+        if (intensity.ptr)
+            check(hipMemsetAsync(intensity.ptr, 0, cells * sizeof(double), stream), "hipMemsetAsync");
+        accumulated = 0;
+    }
+
+    // Host-side injection tables for `steps` steps starting after `first`
+    // (src/gpu.rs:442-465): per step the magnetic ones, then the electric
+    // ones, each sign-scaled, zero values dropped like the reference does.
+    struct StepInjections {
+        uint32_t offset, magnetic, electric;
+    };
+    std::vector<StepInjections> build_injections(uint64_t first, uint64_t steps, int direction,
+                                                 std::vector<InjDesc> &all) {
+        std::vector<StepInjections> table(steps);
+        all.clear();
+        std::vector<InjDesc> electric;
+        for (uint64_t s = 0; s < steps; ++s) {
+            // forward step n → n+1 evaluates at (n+1)·Δt; reverse of step n at n·Δt
+            const uint64_t n = direction > 0 ? first + s + 1 : first - s;
+            const float time = static_cast<float>(n) * time_step;
+            table[s].offset = static_cast<uint32_t>(all.size());
+            electric.clear();
+            for (size_t k = 0; k < sources.size(); ++k) {
+                dp_injection pair[2];
+                source_injections(sources[k], grid, time, pair);
+                for (const dp_injection &q : pair) {
+                    if (q.value == 0.0f) continue;
+                    InjDesc d{};
+                    std::memcpy(d.origin, q.origin, sizeof d.origin);
+                    std::memcpy(d.extent, q.extent, sizeof d.extent);
+                    d.component = q.component;
+                    d.weight_offset = q.inverse_waist_squared == 0.0f ? -1 : source_weight_offset[k];
+                    d.value = static_cast<float>(direction) * q.value;
+                    (q.magnetic ? all : electric).push_back(d);
+                }
+            }
+            table[s].magnetic = static_cast<uint32_t>(all.size()) - table[s].offset;
+            table[s].electric = static_cast<uint32_t>(electric.size());
+            all.insert(all.end(), electric.begin(), electric.end());
+        }
+        return table;
+    }
+
+    void upload_injections(const std::vector<InjDesc> &all) {
+        const size_t bytes = std::max<size_t>(all.size(), 1) * sizeof(InjDesc);
+        // The previous advance may still read the staging or device table.
+        sync();
+        staging.reserve(bytes);
+        if (!all.empty()) std::memcpy(staging.ptr, all.data(), all.size() * sizeof(InjDesc));
+        if (injections.bytes < bytes) injections.allocate(bytes * 2);
+        check(hipMemcpyAsync(injections.ptr, staging.ptr, bytes, hipMemcpyHostToDevice, stream),
+              "hipMemcpyAsync");
+    }
+
+    void fill_tiles(StepArgs &a, int r, uint32_t owned_x) const {
+        const uint32_t nx = grid.extent[0], ny = grid.extent[1], nz = grid.extent[2];
+        a.tiles_x = (nx + owned_x - 1) / owned_x;
+        a.tiles_y = (ny + (r - 2) - 1) / (r - 2);
+        a.depth = std::min<uint32_t>(nz, depth ? depth : 32);
+        a.tiles_z = (nz + a.depth - 1) / a.depth;
+        a.tiles_total = a.tiles_x * a.tiles_y * a.tiles_z;
+        a.xcd_remap = (tile_flags & DP_TILE_XCD_REMAP) ? 1 : 0;
+    }
+
+    void advance(uint64_t steps) {
+        if (steps == 0) return;
+        std::vector<InjDesc> all;
+        auto table = build_injections(step, steps, +1, all);
+        upload_injections(all);
+        const InjDesc *device_inj = injections.as<InjDesc>();
+        if (accumulating) ensure_intensity();
+        const int r = effective_rows();
+        for (uint64_t s = 0; s < steps; ++s) {
+            const auto &t = table[s];
+            if (kernel == DP_KERNEL_FUSED || kernel == DP_KERNEL_FUSED_OVERLAP) {
+                StepArgs a = args(current, 1 - current);
+                a.injections = device_inj + t.offset;
+                a.n_magnetic = t.magnetic;
+                a.n_electric = t.electric;
+                a.intensity = accumulating ? intensity.as<double>() : nullptr;
+                if (kernel == DP_KERNEL_FUSED) {
+                    fill_tiles(a, r, kAlignedX);
+                    launch_fused_aligned(a, r, (tile_flags & DP_TILE_NONTEMPORAL) != 0, stream);
+                } else {
+                    fill_tiles(a, r, kOwnedX);
+                    launch_fused(a, r, stream);
+                }
+                current = 1 - current;
+            } else {
+                StepArgs a = args(current, current);
+                launch_update_h(a, +1, stream);
+                for (uint32_t q = 0; q < t.magnetic; ++q)
+                    launch_inject(h[current].as<float>(), cells, grid.extent[0], grid.extent[1],
+                                  all[t.offset + q], nullptr, weights.as<float>(), stream);
+                launch_update_e(a, +1, stream);
+                for (uint32_t q = 0; q < t.electric; ++q)
+                    launch_inject(e[current].as<float>(), cells, grid.extent[0], grid.extent[1],
+                                  all[t.offset + t.magnetic + q], nullptr, weights.as<float>(),
+                                  stream);
+                if (accumulating) launch_intensity(e[current].as<float>(), intensity.as<double>(), cells, stream);
+            }
+            check(hipGetLastError(), "kernel launch");
+            ++launches;
+            if (accumulating) ++accumulated;
+        }
+        step += steps;
+    }
+
+    void reverse(uint64_t steps) {
+        if (steps == 0) return;
+        if (!reversible)
+            fail(DP_ERR_LOSSY,
+                 "this scene is lossy, and running a dissipative update backwards amplifies "
+                 "roundoff exponentially; reversal needs Boundary::Pec and no conductive material");
+        if (step < steps) fail(DP_ERR_PAST_START, "cannot step back past the start");
+        clear_intensity();
+        std::vector<InjDesc> all;
+        auto table = build_injections(step, steps, -1, all);
+        upload_injections(all);
+        for (uint64_t s = 0; s < steps; ++s) {
+            const auto &t = table[s];
+            StepArgs a = args(current, current);
+            // Forward order is H, inject-H, E, inject-E; backward undoes each
+            // in the opposite order (src/cpu.rs:266-272).
+            for (uint32_t q = 0; q < t.electric; ++q)
+                launch_inject(e[current].as<float>(), cells, grid.extent[0], grid.extent[1],
+                              all[t.offset + t.magnetic + q], nullptr, weights.as<float>(), stream);
+            launch_update_e(a, -1, stream);
+            for (uint32_t q = 0; q < t.magnetic; ++q)
+                launch_inject(h[current].as<float>(), cells, grid.extent[0], grid.extent[1],
+                              all[t.offset + q], nullptr, weights.as<float>(), stream);
+            launch_update_h(a, -1, stream);
+            check(hipGetLastError(), "kernel launch");
+        }
+        step -= steps;
+    }
+};

@@ -516,6 +516,42 @@ static void launch_fused_aligned_as(const StepArgs &args, int rows, bool nontemp
 #undef DPA_ROWS
 }
 
+template <int R>
+static void launch_batch_rows(const InstanceDesc *instances, const StepSlot *slots, const uint32_t *owners,
+                              uint32_t n_tiles, const InjDesc *injections, bool nontemporal, uint32_t blocks,
+                              hipStream_t stream) {
+    if (nontemporal)
+        k_fused_batch<R, true><<<blocks, 256, 0, stream>>>(instances, slots, owners, n_tiles, injections);
+    else
+        k_fused_batch<R, false><<<blocks, 256, 0, stream>>>(instances, slots, owners, n_tiles, injections);
+}
+
+void launch_fused_batch(const InstanceDesc *instances, const StepSlot *slots, const uint32_t *owners,
+                        uint32_t n_tiles, const InjDesc *injections, int rows, bool nontemporal, uint32_t blocks,
+                        hipStream_t stream) {
+    switch (rows) {
+    case 6: launch_batch_rows<6>(instances, slots, owners, n_tiles, injections, nontemporal, blocks, stream); break;
+    case 8: launch_batch_rows<8>(instances, slots, owners, n_tiles, injections, nontemporal, blocks, stream); break;
+    default: launch_batch_rows<4>(instances, slots, owners, n_tiles, injections, nontemporal, blocks, stream); break;
+    }
+}
+
+uint32_t batch_resident_blocks(int rows, bool nontemporal) {
+    int device = 0, per_cu = 0;
+    (void)hipGetDevice(&device);
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) != hipSuccess) return 1024;
+    const void *kernel = nullptr;
+    switch (rows) {
+    case 6: kernel = nontemporal ? (const void *)k_fused_batch<6, true> : (const void *)k_fused_batch<6, false>; break;
+    case 8: kernel = nontemporal ? (const void *)k_fused_batch<8, true> : (const void *)k_fused_batch<8, false>; break;
+    default: kernel = nontemporal ? (const void *)k_fused_batch<4, true> : (const void *)k_fused_batch<4, false>; break;
+    }
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, 256, 0) != hipSuccess || per_cu <= 0)
+        per_cu = 4;
+    return uint32_t(per_cu) * uint32_t(prop.multiProcessorCount);
+}
+
 void launch_fused_aligned(const StepArgs &args, int rows, bool nontemporal, hipStream_t stream) {
     // 32-bit byte offsets while one component fits in 4 GiB (1 Gi cells).
     if (args.cells * sizeof(float) <= 0xFFFFFFFFull)

@@ -57,6 +57,32 @@ struct StepArgs {
     uint32_t xcd_remap;
 };
 
+// One grid of a batch, as the batch kernel sees it (device memory).
+struct alignas(16) InstanceDesc {
+    float *e[2];  // ping-pong pair
+    float *h[2];
+    const uint8_t *material;
+    const float4 *coefficients;
+    const float *axis_table;
+    const float *weights;
+    double *intensity;
+    uint32_t nx, ny, nz, material_count;
+    uint64_t cells;
+    uint32_t tiles_x, tiles_y, tiles_z, depth;
+    uint32_t tiles_total, tile_base, pad0, pad1;
+};
+
+// What one instance does in one batch launch.
+struct alignas(16) StepSlot {
+    uint32_t active;      // steps this launch
+    uint32_t parity;      // which buffer of the pair holds the state
+    uint32_t inj_offset;  // into the launch's injection list
+    uint32_t n_magnetic;
+    uint32_t n_electric;
+    uint32_t accumulate;  // add |E|² into the instance's intensity
+    uint32_t pad[2];
+};
+
 // Tile geometry of the fused kernel: one wave of 64 lanes covers 62 owned
 // columns plus one halo column on each side.
 constexpr int kLanes = 64;
@@ -67,6 +93,13 @@ void launch_fused(const StepArgs &args, int rows, hipStream_t stream);
 // 64-aligned tiles; halo columns fetched by lanes 0 and 63.
 constexpr int kAlignedX = 64;
 void launch_fused_aligned(const StepArgs &args, int rows, bool nontemporal, hipStream_t stream);
+// One launch advancing every active instance of a batch by one step.
+// `blocks` is the persistent grid size (workgroups of 4 waves).
+void launch_fused_batch(const InstanceDesc *instances, const StepSlot *slots, const uint32_t *owners,
+                        uint32_t n_tiles, const InjDesc *injections, int rows, bool nontemporal, uint32_t blocks,
+                        hipStream_t stream);
+// Workgroups of the batch kernel that fit on the device at once.
+uint32_t batch_resident_blocks(int rows, bool nontemporal);
 void launch_update_h(const StepArgs &args, int direction, hipStream_t stream);
 void launch_update_e(const StepArgs &args, int direction, hipStream_t stream);
 void launch_inject(float *field, uint64_t cells, uint32_t nx, uint32_t ny, const InjDesc &desc,

@@ -14,6 +14,10 @@ the workload descriptions:
   lossless palette (u8 indices into 16 materials), a centred z-dipole driven by
   a differentiated Gaussian of σ = 60 Δt, 1000 steps with |E|² accumulated over
   the last 500.
+* ``batched_uneven_256`` — 256 independent solves, each axis drawn
+  log-uniform in [48, 160] cells, half PEC cavities and half absorbing boxes,
+  1–8 dielectric blocks (ε in [1, 4]) each, seeded with a random-phase
+  superposition of 4 low-order discrete cavity modes; 1000 steps.
 """
 from __future__ import annotations
 
@@ -136,6 +140,105 @@ def layered_768(side: int = 768, seed: int = 3, nz: Optional[int] = None) -> Wor
     return Workload("layered_768", scene, 1000, accumulate_from=500,
                     note="40 z-slabs, 15-entry lossless palette (u8 index), centred z-dipole, "
                          "differentiated Gaussian σ=60Δt, intensity over the last 500 steps")
+
+
+@dataclass
+class Instance:
+    """One solve of a batched workload: its scene and initial fields."""
+
+    scene: Scene
+    modes: List[Tuple[Tuple[int, int, int], Tuple[float, float, float], float]]  # (m, amplitude, phase)
+
+    def initial_fields(self) -> Tuple[np.ndarray, np.ndarray]:
+        return cavity_mode_fields(self.scene.grid.extent.as_array(), self.modes)
+
+
+def _mode_factors(n: int, m: int) -> Tuple[np.ndarray, np.ndarray, np.ndarray, np.ndarray]:
+    """1D factors of the discrete PEC mode m along an axis of n cells.
+
+    sin(k·i) samples integer positions (zero on the wall at i = 0 and on the
+    implicit wall at i = n), cos(k·(i+½)) the half positions; the forward
+    differences are those of the H update (the value past the end is 0).
+    """
+    k = math.pi * m / n
+    i = np.arange(n, dtype=np.float64)
+    s = np.sin(k * i)
+    c = np.cos(k * (i + 0.5))
+    ds = np.append(s[1:], 0.0) - s
+    dc = np.append(c[1:], 0.0) - c
+    return s, c, ds, dc
+
+
+def cavity_mode_fields(extent: Tuple[int, int, int], modes) -> Tuple[np.ndarray, np.ndarray]:
+    """E and H (component-major 3N f32) of a superposition of discrete modes.
+
+    Mode (mx, my, mz) has E_a = A_a·cos(k_a(i_a+½))·Π_{b≠a} sin(k_b·i_b) with
+    k_a = π·m_a/n_a; A is projected orthogonal to (sin(k_a/2))_a, which makes
+    the discrete divergence zero. H is the forward-difference curl of E scaled
+    by 1/|K̃|, K̃_a = 2·sin(k_a/2). A mode with phase φ contributes cos φ·E and
+    sin φ·H.
+    """
+    nx, ny, nz = extent
+    n = (nx, ny, nz)
+    terms = {("e", a): [] for a in range(3)}
+    terms.update({("h", a): [] for a in range(3)})
+    for m, amplitude, phase in modes:
+        f = [_mode_factors(n[a], m[a]) for a in range(3)]
+        s_half = np.array([math.sin(math.pi * m[a] / n[a] / 2) for a in range(3)])
+        amp = np.array(amplitude, np.float64)
+        amp = amp - s_half * (amp @ s_half) / (s_half @ s_half)
+        norm = 2.0 * math.sqrt(float(s_half @ s_half))
+        ce, sh = math.cos(phase), math.sin(phase)
+        for a in range(3):
+            # E_a: cos along a, sin along the others
+            terms["e", a].append((ce * amp[a], [f[t][1] if t == a else f[t][0] for t in range(3)]))
+            # H_a = d_b E_c − d_c E_b (forward differences), b = a+1, c = a+2
+            b, c = (a + 1) % 3, (a + 2) % 3
+            fac_c = [f[t][1] if t == c else f[t][0] for t in range(3)]
+            fac_c[b] = f[b][2]
+            fac_b = [f[t][1] if t == b else f[t][0] for t in range(3)]
+            fac_b[c] = f[c][2]
+            terms["h", a].append((sh / norm * amp[c], fac_c))
+            terms["h", a].append((-sh / norm * amp[b], fac_b))
+
+    def assemble(field: str) -> np.ndarray:
+        out = np.empty((3, nz * ny, nx), np.float32)
+        for a in range(3):
+            # Σ_t w_t·fz_t⊗fy_t⊗fx_t as one (nz·ny × T) @ (T × nx) product
+            zy = np.stack([w * np.outer(fz, fy).reshape(-1) for w, (fx, fy, fz) in terms[field, a]], axis=1)
+            xs = np.stack([fx for _, (fx, _, _) in terms[field, a]], axis=0)
+            out[a] = zy @ xs
+        return out.reshape(-1)
+
+    return assemble("e"), assemble("h")
+
+
+def batched_uneven(count: int = 256, seed: int = 4, lo: int = 48, hi: int = 160,
+                   cell: float = 1e-3) -> List[Instance]:
+    rng = SplitMix64(seed)
+    out: List[Instance] = []
+    for _ in range(count):
+        extent = Extent(rng.log_uniform_int(lo, hi), rng.log_uniform_int(lo, hi), rng.log_uniform_int(lo, hi))
+        grid = Grid(extent, cell)
+        boundary = Boundary.PEC if rng.integer(0, 1) == 0 else Boundary.DEFAULT
+        scene = Scene(grid, boundary)
+        size = grid.size()
+        for _ in range(rng.integer(1, 8)):
+            m = scene.materials.push(Material.dielectric(rng.uniform(1.0, 4.0)))
+            dims = [rng.uniform(0.1, 0.5) * s for s in size]
+            centre = [rng.uniform(-0.5 * s + 0.5 * d, 0.5 * s - 0.5 * d) for s, d in zip(size, dims)]
+            scene.shapes.append(Shape.Block(centre, dims, m))
+        modes = []
+        for _ in range(4):
+            mode = (rng.integer(1, 3), rng.integer(1, 3), rng.integer(1, 3))
+            amplitude = (rng.uniform(-1, 1), rng.uniform(-1, 1), rng.uniform(-1, 1))
+            modes.append((mode, amplitude, rng.uniform(0.0, 2.0 * math.pi)))
+        out.append(Instance(scene, modes))
+    return out
+
+
+def batched_uneven_256(count: int = 256, seed: int = 4) -> List[Instance]:
+    return batched_uneven(count, seed)
 
 
 WORKLOADS: Dict[str, Callable[..., Workload]] = {

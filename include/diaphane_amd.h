@@ -298,6 +298,52 @@ int dp_sim_stream_probe(dp_sim *sim, int iterations, double *gbs);
 /* Forward-step launches enqueued so far (fused or H-kernel launches). */
 uint64_t dp_sim_launch_count(const dp_sim *sim);
 
+/* ------------------------------------------------------------------------ */
+/* Batches: many independent sims advanced by one launch per step           */
+/* ------------------------------------------------------------------------ */
+/* The reference runs one gpu::Simulation per scene, N steps per command
+ * buffer (src/gpu.rs:393-484); it has no batched form. A dp_batch advances
+ * its member sims together: every launch does one step of every active
+ * member over a packed table of tiles, so small grids share the device
+ * instead of each under-filling it. Each member's fields are bit-identical
+ * to the same sim advanced alone.
+ *
+ * A member keeps every dp_sim read/write/reset/energy entry point; its work
+ * is ordered on the batch's stream. dp_sim_advance and dp_sim_reverse refuse
+ * a member. Destroying a member sim removes it from its batch; destroying a
+ * batch releases its members (their state stays as it is). A batch and its
+ * members are externally synchronized together. A member's field component
+ * must fit in 4 GiB (cells ≤ 2^30). */
+typedef struct dp_batch dp_batch;
+int dp_batch_create(dp_engine *engine, dp_batch **out);
+void dp_batch_destroy(dp_batch *batch);
+int dp_batch_add(dp_batch *batch, dp_sim *sim);
+int dp_batch_remove(dp_batch *batch, dp_sim *sim);
+/* Step count at which a member stops; members without one run with every
+ * advance. A member at its target sits out further launches. */
+int dp_batch_set_target(dp_batch *batch, dp_sim *sim, uint64_t step);
+/* Enqueues `steps` launches (fewer if every member reaches its target),
+ * each advancing every member below its target by one step. Asynchronous. */
+int dp_batch_advance(dp_batch *batch, uint64_t steps);
+int dp_batch_wait(dp_batch *batch, uint32_t timeout_ms);
+/* Launches per captured HIP graph (default 32; 0 launches each kernel
+ * directly). Results never depend on it. */
+int dp_batch_set_graph_steps(dp_batch *batch, uint32_t steps);
+/* Rows per wave (4, 6 or 8; 0 = default 4), z-slab depth (0 = 32) and
+ * DP_TILE_NONTEMPORAL (flags < 0 keeps the current flags). */
+int dp_batch_set_tile(dp_batch *batch, int rows, int depth, int flags);
+/* Autotuner: times `trials` launches of every rows × depth candidate
+ * (rows 4, 6, 8; depth 8, 16, 32, 64) on the members as they are, keeps the
+ * fastest and reports it (any out pointer may be null). Member state is
+ * unchanged: trial launches write only the idle half of each ping-pong
+ * pair. Synchronous. */
+int dp_batch_autotune(dp_batch *batch, uint32_t trials, int *rows, int *depth, float *ms_per_step);
+uint32_t dp_batch_size(const dp_batch *batch);
+uint64_t dp_batch_cells(const dp_batch *batch);
+uint64_t dp_batch_launch_count(const dp_batch *batch);
+int dp_batch_record_event(dp_batch *batch, int slot);
+int dp_batch_elapsed_ms(dp_batch *batch, int from_slot, int to_slot, float *ms);
+
 #ifdef __cplusplus
 }
 #endif
