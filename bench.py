@@ -15,6 +15,12 @@ log-uniform in [48, 160] per axis, PEC or absorbing, seeded cavity modes),
 1000 steps each, advanced one launch per step for the whole batch; the same
 solves run one at a time on the same GPU are timed beside it, and a sample of
 members is checked bit for bit between the two.
+
+``--workload served_stream``: Poisson arrivals (``--rate`` req/s) of solve
+requests of mixed kinds, sizes and step counts, served by the continuous-
+batching request loop under a 200 GiB device-memory budget, then the same
+stream served one request at a time; reports aggregate cell-updates/s and
+p50/p99 completion latency for both.
 """
 from __future__ import annotations
 
@@ -23,6 +29,8 @@ import json
 import os
 import sys
 import time
+
+import numpy as np
 
 HBM_PEAK_GBS = 8000.0          # MI355X HBM3E, GB/s
 BYTES_PER_CELL = 49            # 6 f32 fields read+written + 1 u8 material index
@@ -228,6 +236,104 @@ def bench_batched(args) -> int:
     return 0
 
 
+def serve_stream(context, stream, one_at_a_time: bool, chunk_steps: int, rows: int, depth: int) -> dict:
+    """Feeds the stream to a server at its arrival times; returns per-request
+    latencies and the aggregate rate."""
+    from diaphane_amd.serve import DONE, Server
+
+    server = Server(context, one_at_a_time=one_at_a_time, chunk_steps=chunk_steps, rows=rows, depth=depth,
+                    host_threads=4)
+    pending = {}
+    infos = []
+    peak_bytes = 0
+    start = time.perf_counter()
+    next_log = start + 10.0
+    i = 0
+    while i < len(stream) or pending:
+        now = time.perf_counter() - start
+        while i < len(stream) and stream[i].arrival_s <= now:
+            pending[server.submit(stream[i].scene, stream[i].steps)] = i
+            i += 1
+        peak_bytes = max(peak_bytes, server.stats()["used_bytes"])
+        for ticket in list(pending):
+            info = server.info(ticket)
+            if info.state == DONE:
+                server.wait(ticket)  # raises if the request failed
+                infos.append((pending.pop(ticket), info))
+                server.release(ticket)
+        if time.perf_counter() >= next_log:
+            log(f"  {len(infos)}/{len(stream)} done, {len(pending)} in flight")
+            next_log += 10.0
+        wait = 0.001 if i >= len(stream) else min(0.001, max(0.0, stream[i].arrival_s - now))
+        time.sleep(wait)
+    stats = server.stats()
+    server.close()
+    infos.sort()
+    latencies = np.array([info.latency_s for _, info in infos])
+    first = min(info.submitted_s for _, info in infos)
+    last = max(info.completed_s for _, info in infos)
+    updates = sum(info.cells * info.steps for _, info in infos)
+    return {
+        "value": updates / (last - first),
+        "unit": "cell-updates/s",
+        "makespan_s": last - first,
+        "latency_s": {"p50": float(np.percentile(latencies, 50)), "p99": float(np.percentile(latencies, 99)),
+                      "mean": float(latencies.mean()), "max": float(latencies.max())},
+        "peak_resident_bytes": peak_bytes,
+        "chunks": stats["chunks"],
+        "device_busy_s": stats["busy_s"],
+    }
+
+
+def bench_served(args) -> int:
+    from diaphane_amd import workloads
+    from diaphane_amd.gpu import headless_context
+
+    stream = workloads.served_stream(args.requests, args.rate)
+    context = headless_context()
+    updates = sum(r.scene.grid.extent.total() * r.steps for r in stream)
+    log(f"{len(stream)} requests over {stream[-1].arrival_s:.1f} s, {updates / 1e9:.0f} G cell-updates")
+    # warm the code paths (kernels load on first launch)
+    warm = workloads.served_stream(3, 1000.0, seed=99, lo=32, hi=40, steps_lo=40, steps_hi=40)
+    serve_stream(context, warm, False, args.chunk_steps, args.rows, 0)
+    serve_stream(context, warm, True, args.chunk_steps, args.rows, 0)
+    log("continuous batching")
+    batched = serve_stream(context, stream, False, args.chunk_steps, args.rows, 0)
+    log(f"  {batched['value'] / 1e9:.1f} G/s, p50 {batched['latency_s']['p50']:.2f} s")
+    log("one at a time")
+    sequential = serve_stream(context, stream, True, args.chunk_steps, args.rows, 0)
+    log(f"  {sequential['value'] / 1e9:.1f} G/s, p50 {sequential['latency_s']['p50']:.2f} s")
+    kinds = {k: sum(1 for r in stream if r.kind == k) for k in workloads.SERVED_KINDS}
+    result = {
+        "metric": "served aggregate Yee cell-updates/s with p50/p99 completion latency",
+        "value": batched["value"],
+        "unit": "cell-updates/s",
+        "n_gpus": 1,
+        "steps": int(sum(r.steps for r in stream)),
+        "warmup": 0,
+        "ms_per_step": None,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": "served_stream",
+            "requests": len(stream),
+            "rate_per_s": args.rate,
+            "cell_updates": updates,
+            "kinds": kinds,
+            "hbm_budget_bytes": 200 << 30,
+            "chunk_steps": args.chunk_steps,
+        },
+        "served": batched,
+        "sequential": sequential,
+        "speedup_vs_sequential": batched["value"] / sequential["value"],
+    }
+    print(json.dumps(result), flush=True)
+    return 0
+
+
 def main() -> int:
     parser = argparse.ArgumentParser(description=__doc__)
     parser.add_argument("--gpus", type=int, default=1)
@@ -237,6 +343,9 @@ def main() -> int:
     parser.add_argument("--rows", type=int, default=0, help="fused-kernel tile rows (0 = default)")
     parser.add_argument("--no-cpu", action="store_true", help="skip the CPU-oracle baseline")
     parser.add_argument("--cpu-seconds", type=float, default=10.0)
+    parser.add_argument("--requests", type=int, default=200, help="served_stream: number of requests")
+    parser.add_argument("--rate", type=float, default=50.0, help="served_stream: arrivals per second")
+    parser.add_argument("--chunk-steps", type=int, default=32, help="served_stream: launches between joins")
     parser.add_argument("--intensity-from", type=int, default=None,
                         help="step at which |E|² accumulation starts (-1: never; default: the workload's)")
     args = parser.parse_args()
@@ -248,6 +357,8 @@ def main() -> int:
 
     if args.workload == "batched_uneven_256":
         return bench_batched(args)
+    if args.workload == "served_stream":
+        return bench_served(args)
 
     from diaphane_amd import workloads
     from diaphane_amd.gpu import Simulation, headless_context

@@ -11,6 +11,24 @@ from pathlib import Path
 
 from .scene import DpGridSpec, DpInjection, DpScene
 
+
+class DpServerConfig(ctypes.Structure):
+    _fields_ = [("hbm_budget_bytes", c_uint64), ("max_resident", c_uint32), ("chunk_steps", c_uint32),
+                ("host_threads", c_uint32), ("one_at_a_time", ctypes.c_int32), ("rows", ctypes.c_int32),
+                ("depth", ctypes.c_int32)]
+
+
+class DpRequestInfo(ctypes.Structure):
+    _fields_ = [("state", ctypes.c_int32), ("status", ctypes.c_int32), ("steps", c_uint64), ("step", c_uint64),
+                ("cells", c_uint64), ("footprint_bytes", c_uint64), ("submitted_s", c_double),
+                ("admitted_s", c_double), ("started_s", c_double), ("completed_s", c_double)]
+
+
+class DpServerStats(ctypes.Structure):
+    _fields_ = [("queued", c_uint32), ("preparing", c_uint32), ("ready", c_uint32), ("running", c_uint32),
+                ("used_bytes", c_uint64), ("budget_bytes", c_uint64), ("chunks", c_uint64), ("busy_s", c_double),
+                ("uptime_s", c_double)]
+
 LIBRARY_PATH = Path(__file__).resolve().parent / "libdiaphane_amd.so"
 
 DP_OK = 0
@@ -92,6 +110,14 @@ def _declare(lib: ctypes.CDLL) -> None:
         "dp_batch_launch_count": (c_uint64, [c_void_p]),
         "dp_batch_record_event": (c_int, [c_void_p, c_int]),
         "dp_batch_elapsed_ms": (c_int, [c_void_p, c_int, c_int, POINTER(c_float)]),
+        "dp_server_create": (c_int, [c_void_p, POINTER(DpServerConfig), POINTER(c_void_p)]),
+        "dp_server_destroy": (None, [c_void_p]),
+        "dp_server_submit": (c_int, [c_void_p, POINTER(DpScene), c_uint64, POINTER(c_uint64)]),
+        "dp_server_query": (c_int, [c_void_p, c_uint64, POINTER(DpRequestInfo)]),
+        "dp_server_wait": (c_int, [c_void_p, c_uint64, c_uint32]),
+        "dp_server_result": (c_int, [c_void_p, c_uint64, POINTER(c_void_p)]),
+        "dp_server_release": (c_int, [c_void_p, c_uint64]),
+        "dp_server_statistics": (c_int, [c_void_p, POINTER(DpServerStats)]),
     }
     for name, (restype, argtypes) in table.items():
         fn = getattr(lib, name)

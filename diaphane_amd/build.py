@@ -37,6 +37,7 @@ SOURCES = [
     ("scene.cpp", HOST),
     ("engine.cpp", HOST),
     ("batch.cpp", HOST),
+    ("server.cpp", HOST),
 ]
 
 

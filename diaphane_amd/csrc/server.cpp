@@ -1,0 +1,375 @@
+// The served form: a request loop over one batch with continuous batching.
+//
+// Clients submit solves (a scene and a step count) from any thread. A
+// request is admitted when its device footprint fits the HBM budget; host
+// workers then build its sim (rasterization, tables, uploads) while the GPU
+// keeps stepping the others. Between chunks of launches the loop adds
+// prepared sims to the batch and retires the ones that reached their step
+// count, so requests that arrive while others run join the next launch
+// instead of waiting for the device to drain. Finished sims stay resident
+// until the client releases them.
+#include <condition_variable>
+#include <deque>
+#include <mutex>
+#include <unordered_map>
+
+#include "engine.hpp"
+
+namespace {
+
+using Clock = std::chrono::steady_clock;
+
+// A scene with its arrays copied, since the caller's may be gone by the
+// time a worker builds the sim.
+struct OwnedScene {
+    dp_scene scene{};
+    std::vector<dp_refinement> refinements;
+    std::vector<dp_material> materials;
+    std::vector<dp_shape> shapes;
+    std::vector<dp_source> sources;
+
+    explicit OwnedScene(const dp_scene &from) : scene(from) {
+        refinements.assign(from.grid.refinements, from.grid.refinements + from.grid.refinement_count);
+        materials.assign(from.materials, from.materials + from.material_count);
+        shapes.assign(from.shapes, from.shapes + from.shape_count);
+        sources.assign(from.sources, from.sources + from.source_count);
+        scene.grid.refinements = refinements.data();
+        scene.materials = materials.data();
+        scene.shapes = shapes.data();
+        scene.sources = sources.data();
+    }
+};
+
+struct Request {
+    uint64_t ticket = 0;
+    std::unique_ptr<OwnedScene> scene;
+    uint64_t steps = 0;
+    uint64_t cells = 0;
+    uint64_t footprint = 0;
+    int state = DP_REQUEST_QUEUED;
+    int status = DP_OK;
+    std::string error;
+    dp_sim *sim = nullptr;
+    uint64_t step = 0;  // progress as of the last finished chunk
+    double submitted = 0, admitted = 0, started = 0, completed = 0;
+};
+
+// Device bytes a sim of `cells` cells holds: two E and two H buffers of
+// 12 bytes per cell (ping-pong), the u8 material index, and a small fixed
+// allowance for tables, injection lists and energy partials.
+uint64_t footprint_of(uint64_t cells, const dp_scene &scene) {
+    const uint64_t fixed = (1u << 20) + uint64_t(scene.material_count) * 16;
+    return cells * (4 * 12 + 1) + fixed;
+}
+
+}  // namespace
+
+struct dp_server {
+    dp_engine *engine = nullptr;
+    dp_server_config config{};
+    dp_batch *batch = nullptr;
+    Clock::time_point origin = Clock::now();
+
+    std::mutex mu;
+    std::condition_variable wake_loop, wake_workers, finished;
+    std::unordered_map<uint64_t, std::unique_ptr<Request>> requests;
+    std::deque<Request *> queued;     // waiting for budget
+    std::deque<Request *> preparing;  // admitted, waiting for a worker
+    std::deque<Request *> ready;      // sim built, waiting to join
+    std::vector<Request *> running;   // touched by the loop thread only
+    uint32_t running_count = 0;       // its size, for readers under the lock
+    uint64_t used_bytes = 0;
+    uint32_t resident = 0;  // admitted and not finished
+    uint64_t next_ticket = 1;
+    uint64_t chunks = 0;
+    double busy_seconds = 0;
+    bool stopping = false;
+    std::string loop_error;
+
+    std::thread loop;
+    std::vector<std::thread> workers;
+
+    double now() const { return std::chrono::duration<double>(Clock::now() - origin).count(); }
+
+    // Moves queued requests to the workers while the budget allows (FIFO:
+    // a request that does not fit holds back the ones behind it).
+    void admit_locked() {
+        while (!queued.empty()) {
+            Request *r = queued.front();
+            if (used_bytes + r->footprint > config.hbm_budget_bytes) break;
+            if (resident >= config.max_resident) break;
+            queued.pop_front();
+            used_bytes += r->footprint;
+            ++resident;
+            r->state = DP_REQUEST_PREPARING;
+            r->admitted = now();
+            preparing.push_back(r);
+            wake_workers.notify_one();
+        }
+    }
+
+    void finish_locked(Request *r, int status, const std::string &error) {
+        r->state = DP_REQUEST_DONE;
+        r->status = status;
+        r->error = error;
+        r->completed = now();
+        --resident;
+        if (status != DP_OK && r->sim) {
+            dp_sim_destroy(r->sim);
+            r->sim = nullptr;
+        }
+        if (status != DP_OK) used_bytes -= r->footprint;
+        admit_locked();
+        finished.notify_all();
+    }
+
+    void worker_main() {
+        (void)hipSetDevice(engine->device);
+        for (;;) {
+            Request *r = nullptr;
+            {
+                std::unique_lock<std::mutex> lock(mu);
+                wake_workers.wait(lock, [&] { return stopping || !preparing.empty(); });
+                if (stopping) return;
+                r = preparing.front();
+                preparing.pop_front();
+            }
+            dp_sim *sim = nullptr;
+            const int status = dp_sim_create(engine, &r->scene->scene, &sim);
+            const std::string error = status == DP_OK ? std::string() : last_error();
+            std::lock_guard<std::mutex> lock(mu);
+            if (status != DP_OK) {
+                finish_locked(r, status, error);
+                continue;
+            }
+            r->sim = sim;
+            r->state = DP_REQUEST_READY;
+            ready.push_back(r);
+            wake_loop.notify_one();
+        }
+    }
+
+    void loop_main() {
+        (void)hipSetDevice(engine->device);
+        for (;;) {
+            std::vector<Request *> joining;
+            {
+                std::unique_lock<std::mutex> lock(mu);
+                wake_loop.wait(lock, [&] { return stopping || !ready.empty() || !running.empty(); });
+                if (stopping) return;
+                if (config.one_at_a_time) {
+                    if (running.empty() && !ready.empty()) {
+                        joining.push_back(ready.front());
+                        ready.pop_front();
+                    }
+                } else {
+                    joining.assign(ready.begin(), ready.end());
+                    ready.clear();
+                }
+                for (Request *r : joining) {
+                    r->state = DP_REQUEST_RUNNING;
+                    r->started = now();
+                }
+            }
+            const auto t0 = Clock::now();
+            int status = DP_OK;
+            std::string error;
+            if (config.one_at_a_time) {
+                running.insert(running.end(), joining.begin(), joining.end());
+                Request *r = running.front();
+                status = dp_sim_advance(r->sim, r->steps - r->sim->step);
+                if (status == DP_OK) status = dp_sim_wait(r->sim, 0);
+            } else {
+                for (Request *r : joining) {
+                    status = dp_batch_add(batch, r->sim);
+                    if (status == DP_OK) status = dp_batch_set_target(batch, r->sim, r->steps);
+                    if (status != DP_OK) {
+                        error = last_error();
+                        std::lock_guard<std::mutex> lock(mu);
+                        finish_locked(r, status, error);
+                        status = DP_OK;
+                        continue;
+                    }
+                    running.push_back(r);
+                }
+                if (!running.empty()) {
+                    status = dp_batch_advance(batch, config.chunk_steps);
+                    if (status == DP_OK) status = dp_batch_wait(batch, 0);
+                }
+            }
+            if (status != DP_OK) error = last_error();
+            const double busy = std::chrono::duration<double>(Clock::now() - t0).count();
+
+            std::vector<Request *> done;
+            std::vector<Request *> still;
+            for (Request *r : running) (r->sim->step >= r->steps || status != DP_OK ? done : still).push_back(r);
+            for (Request *r : done)
+                if (!config.one_at_a_time) (void)dp_batch_remove(batch, r->sim);
+            std::lock_guard<std::mutex> lock(mu);
+            ++chunks;
+            busy_seconds += busy;
+            if (status != DP_OK && loop_error.empty()) loop_error = error;
+            for (Request *r : still) r->step = r->sim->step;
+            for (Request *r : done) {
+                r->step = r->sim->step;
+                finish_locked(r, status, error);
+            }
+            running.swap(still);
+            running_count = static_cast<uint32_t>(running.size());
+        }
+    }
+
+    Request *find_locked(uint64_t ticket) {
+        auto it = requests.find(ticket);
+        if (it == requests.end()) fail(DP_ERR_INVALID, "unknown ticket");
+        return it->second.get();
+    }
+
+    ~dp_server() {
+        {
+            std::lock_guard<std::mutex> lock(mu);
+            stopping = true;
+        }
+        wake_loop.notify_all();
+        wake_workers.notify_all();
+        if (loop.joinable()) loop.join();
+        for (auto &w : workers)
+            if (w.joinable()) w.join();
+        if (batch) dp_batch_destroy(batch);
+        for (auto &entry : requests)
+            if (entry.second->sim) dp_sim_destroy(entry.second->sim);
+    }
+};
+
+extern "C" {
+
+int dp_server_create(dp_engine *engine, const dp_server_config *config, dp_server **out) {
+    return guarded([&] {
+        if (!engine || !out) fail(DP_ERR_INVALID, "null argument");
+        auto server = std::make_unique<dp_server>();
+        server->engine = engine;
+        dp_server_config c{};
+        if (config) c = *config;
+        if (c.hbm_budget_bytes == 0) c.hbm_budget_bytes = 200ull << 30;
+        if (c.max_resident == 0) c.max_resident = 1024;
+        if (c.chunk_steps == 0) c.chunk_steps = 32;
+        if (c.host_threads == 0) c.host_threads = 4;
+        if (c.host_threads > 64) fail(DP_ERR_INVALID, "at most 64 host threads");
+        server->config = c;
+        if (!c.one_at_a_time) {
+            if (dp_batch_create(engine, &server->batch) != DP_OK) throw Error{DP_ERR_DEVICE, last_error()};
+            if (dp_batch_set_graph_steps(server->batch, c.chunk_steps <= 4096 ? c.chunk_steps : 0) != DP_OK ||
+                dp_batch_set_tile(server->batch, c.rows, c.depth, -1) != DP_OK)
+                throw Error{DP_ERR_INVALID, last_error()};
+        }
+        dp_server *s = server.get();
+        s->loop = std::thread([s] { s->loop_main(); });
+        for (uint32_t i = 0; i < c.host_threads; ++i) s->workers.emplace_back([s] { s->worker_main(); });
+        *out = server.release();
+    });
+}
+
+void dp_server_destroy(dp_server *server) { delete server; }
+
+int dp_server_submit(dp_server *server, const dp_scene *scene, uint64_t steps, uint64_t *ticket) {
+    return guarded([&] {
+        if (!server || !scene || !ticket) fail(DP_ERR_INVALID, "null argument");
+        if (scene->material_count && !scene->materials) fail(DP_ERR_INVALID, "materials pointer is null");
+        if (scene->shape_count && !scene->shapes) fail(DP_ERR_INVALID, "shapes pointer is null");
+        if (scene->source_count && !scene->sources) fail(DP_ERR_INVALID, "sources pointer is null");
+        if (scene->grid.refinement_count && !scene->grid.refinements)
+            fail(DP_ERR_INVALID, "refinements pointer is null");
+        auto r = std::make_unique<Request>();
+        r->scene = std::make_unique<OwnedScene>(*scene);
+        r->cells = Grid::resolve(scene->grid).cells();  // validates the grid
+        r->steps = steps;
+        r->footprint = footprint_of(r->cells, *scene);
+        std::lock_guard<std::mutex> lock(server->mu);
+        if (r->footprint > server->config.hbm_budget_bytes)
+            fail(DP_ERR_BUDGET, "the request alone exceeds the device-memory budget");
+        r->ticket = server->next_ticket++;
+        r->submitted = server->now();
+        *ticket = r->ticket;
+        server->queued.push_back(r.get());
+        server->requests.emplace(r->ticket, std::move(r));
+        server->admit_locked();
+    });
+}
+
+int dp_server_query(dp_server *server, uint64_t ticket, dp_request_info *info) {
+    return guarded([&] {
+        if (!server || !info) fail(DP_ERR_INVALID, "null argument");
+        std::lock_guard<std::mutex> lock(server->mu);
+        const Request *r = server->find_locked(ticket);
+        info->state = r->state;
+        info->status = r->status;
+        info->steps = r->steps;
+        info->step = r->step;
+        info->cells = r->cells;
+        info->footprint_bytes = r->footprint;
+        info->submitted_s = r->submitted;
+        info->admitted_s = r->admitted;
+        info->started_s = r->started;
+        info->completed_s = r->completed;
+    });
+}
+
+int dp_server_wait(dp_server *server, uint64_t ticket, uint32_t timeout_ms) {
+    return guarded([&] {
+        if (!server) fail(DP_ERR_INVALID, "null server");
+        std::unique_lock<std::mutex> lock(server->mu);
+        Request *r = server->find_locked(ticket);
+        const auto limit = std::chrono::milliseconds(timeout_ms ? timeout_ms : 600000);
+        if (!server->finished.wait_for(lock, limit, [&] { return r->state == DP_REQUEST_DONE; }))
+            fail(DP_ERR_TIMEOUT, "the request did not finish in time");
+        if (r->status != DP_OK) throw Error{r->status, r->error};
+    });
+}
+
+int dp_server_result(dp_server *server, uint64_t ticket, dp_sim **sim) {
+    return guarded([&] {
+        if (!server || !sim) fail(DP_ERR_INVALID, "null argument");
+        std::lock_guard<std::mutex> lock(server->mu);
+        Request *r = server->find_locked(ticket);
+        if (r->state != DP_REQUEST_DONE) fail(DP_ERR_INVALID, "the request has not finished");
+        if (r->status != DP_OK) throw Error{r->status, r->error};
+        *sim = r->sim;
+    });
+}
+
+int dp_server_release(dp_server *server, uint64_t ticket) {
+    return guarded([&] {
+        if (!server) fail(DP_ERR_INVALID, "null server");
+        std::unique_ptr<Request> owned;
+        {
+            std::lock_guard<std::mutex> lock(server->mu);
+            Request *r = server->find_locked(ticket);
+            if (r->state != DP_REQUEST_DONE) fail(DP_ERR_INVALID, "the request has not finished");
+            auto it = server->requests.find(ticket);
+            owned = std::move(it->second);
+            server->requests.erase(it);
+            if (owned->status == DP_OK) server->used_bytes -= owned->footprint;
+        }
+        if (owned->sim) dp_sim_destroy(owned->sim);
+        std::lock_guard<std::mutex> lock(server->mu);
+        server->admit_locked();
+    });
+}
+
+int dp_server_statistics(dp_server *server, dp_server_stats *stats) {
+    return guarded([&] {
+        if (!server || !stats) fail(DP_ERR_INVALID, "null argument");
+        std::lock_guard<std::mutex> lock(server->mu);
+        stats->queued = static_cast<uint32_t>(server->queued.size());
+        stats->preparing = static_cast<uint32_t>(server->preparing.size());
+        stats->ready = static_cast<uint32_t>(server->ready.size());
+        stats->running = server->running_count;
+        stats->used_bytes = server->used_bytes;
+        stats->budget_bytes = server->config.hbm_budget_bytes;
+        stats->chunks = server->chunks;
+        stats->busy_s = server->busy_seconds;
+        stats->uptime_s = server->now();
+    });
+}
+
+}  // extern "C"

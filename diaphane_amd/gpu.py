@@ -99,11 +99,27 @@ class Simulation:
         self.extent = (int(extent[0]), int(extent[1]), int(extent[2]))
         self.cells = self.extent[0] * self.extent[1] * self.extent[2]
 
+    @classmethod
+    def borrowed(cls, handle: c_void_p, grid=None) -> "Simulation":
+        """A view of a sim owned elsewhere (a served request's result); closing
+        it leaves the sim alone."""
+        self = cls.__new__(cls)
+        self._context = None
+        self._scene_c = None
+        self._handle = handle
+        self._owned = False
+        self.grid = grid
+        extent = (c_uint32 * 3)()
+        check(lib().dp_sim_extent(self._handle, extent))
+        self.extent = (int(extent[0]), int(extent[1]), int(extent[2]))
+        self.cells = self.extent[0] * self.extent[1] * self.extent[2]
+        return self
+
     def close(self) -> None:
         handle = getattr(self, "_handle", None)
-        if handle:
+        if handle and getattr(self, "_owned", True):
             lib().dp_sim_destroy(handle)
-            self._handle = None
+        self._handle = None
 
     def __del__(self):
         try:

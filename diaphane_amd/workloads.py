@@ -18,6 +18,10 @@ the workload descriptions:
   log-uniform in [48, 160] cells, half PEC cavities and half absorbing boxes,
   1–8 dielectric blocks (ε in [1, 4]) each, seeded with a random-phase
   superposition of 4 low-order discrete cavity modes; 1000 steps.
+* ``served_stream`` — Poisson arrivals of solve requests, each axis drawn
+  log-uniform in [32, 256] cells, step counts uniform in [200, 5000], scene
+  types mixed evenly: PEC cavity ringdown, TFSF plane wave on a PEC sphere
+  of random radius, and a matched-lossy half-space with σ in [0.01, 0.1].
 """
 from __future__ import annotations
 
@@ -239,6 +243,64 @@ def batched_uneven(count: int = 256, seed: int = 4, lo: int = 48, hi: int = 160,
 
 def batched_uneven_256(count: int = 256, seed: int = 4) -> List[Instance]:
     return batched_uneven(count, seed)
+
+
+@dataclass
+class Request:
+    """One request of a served stream."""
+
+    arrival_s: float
+    kind: str
+    scene: Scene
+    steps: int
+
+
+SERVED_KINDS = ("cavity_ringdown", "tfsf_sphere", "lossy_half_space")
+
+
+def served_scene(kind: str, extent: Extent, rng: SplitMix64, cell: float = 1e-3) -> Scene:
+    grid = Grid(extent, cell)
+    size = grid.size()
+    if kind == "cavity_ringdown":
+        # a Ricker kick in a closed PEC box, then free ringing
+        scene = Scene(grid, Boundary.PEC)
+        at = [rng.uniform(-0.3, 0.3) * s for s in size]
+        scene.sources.append(Source.point(at, Axis.Z, Waveform.ricker(grid.frequency_for_resolution(16.0))))
+        return scene
+    if kind == "tfsf_sphere":
+        scene = Scene(grid, Boundary.DEFAULT)
+        pec = scene.materials.push(Material.PERFECT_CONDUCTOR)
+        radius = rng.uniform(0.1, 0.3) * min(size)
+        scene.shapes.append(Shape.Sphere([0.0, 0.0, 0.0], radius, pec))
+        offset = -0.5 * size[0] + (Boundary.DEFAULT.thickness + 2) * cell
+        scene.sources.append(Source.plane_wave(Axis.X, offset, Axis.Z, Waveform.gaussian_pulse(
+            grid.frequency_for_resolution(20.0), 3.0)))
+        return scene
+    if kind == "lossy_half_space":
+        scene = Scene(grid, Boundary.DEFAULT)
+        lossy = scene.materials.push(Material.matched_lossy(rng.uniform(1.0, 4.0), rng.uniform(0.01, 0.1)))
+        scene.shapes.append(Shape.Block([0.0, 0.0, -0.25 * size[2]], [2.0 * size[0], 2.0 * size[1], 0.5 * size[2]],
+                                        lossy))
+        scene.sources.append(Source.point([0.0, 0.0, 0.25 * size[2]], Axis.Z,
+                                          Waveform.ricker(grid.frequency_for_resolution(16.0))))
+        return scene
+    raise ValueError(f"unknown request kind {kind!r}")
+
+
+def served_stream(count: int = 200, rate: float = 50.0, seed: int = 5, lo: int = 32, hi: int = 256,
+                  steps_lo: int = 200, steps_hi: int = 5000) -> List[Request]:
+    """``count`` requests with exponential inter-arrival gaps of mean 1/rate;
+    kinds cycle so the mix is even."""
+    rng = SplitMix64(seed)
+    t = 0.0
+    out: List[Request] = []
+    for k in range(count):
+        t += -math.log(1.0 - rng.uniform()) / rate
+        extent = Extent(rng.log_uniform_int(lo, hi), rng.log_uniform_int(lo, hi), rng.log_uniform_int(lo, hi))
+        kind = SERVED_KINDS[k % len(SERVED_KINDS)]
+        scene = served_scene(kind, extent, rng)
+        out.append(Request(t, kind, scene, rng.integer(steps_lo, steps_hi)))
+    return out
 
 
 WORKLOADS: Dict[str, Callable[..., Workload]] = {

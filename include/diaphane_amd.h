@@ -344,6 +344,56 @@ uint64_t dp_batch_launch_count(const dp_batch *batch);
 int dp_batch_record_event(dp_batch *batch, int slot);
 int dp_batch_elapsed_ms(dp_batch *batch, int from_slot, int to_slot, float *ms);
 
+/* ------------------------------------------------------------------------ */
+/* Serving: a request loop with continuous batching                         */
+/* ------------------------------------------------------------------------ */
+/* No counterpart in the reference (one gpu::Simulation per caller,
+ * src/gpu.rs:190). A dp_server owns a batch, a loop thread and host worker
+ * threads. Clients submit solves from any thread; a request is admitted
+ * when its device footprint (49 bytes per cell plus a small fixed part)
+ * fits the HBM budget, built by a worker while the device keeps stepping
+ * the others, and joins the batch between chunks of launches. A finished
+ * request's sim stays resident, readable through the dp_sim entry points,
+ * until dp_server_release. Results are bit-identical to the same scene run
+ * alone through dp_sim_advance. */
+typedef struct dp_server dp_server;
+typedef struct dp_server_config {
+    uint64_t hbm_budget_bytes; /* 0: 200 GiB                              */
+    uint32_t max_resident;     /* admitted, unfinished requests (0: 1024) */
+    uint32_t chunk_steps;      /* launches between joins/retires (0: 32)  */
+    uint32_t host_threads;     /* sim builders (0: 4)                     */
+    int32_t one_at_a_time;     /* 1: run requests alone, in arrival order,
+                                  through dp_sim_advance (the baseline)   */
+    int32_t rows, depth;       /* batch tiling (0: defaults)              */
+} dp_server_config;
+enum { DP_REQUEST_QUEUED = 0, DP_REQUEST_PREPARING = 1, DP_REQUEST_READY = 2, DP_REQUEST_RUNNING = 3,
+       DP_REQUEST_DONE = 4 };
+typedef struct dp_request_info {
+    int32_t state;  /* DP_REQUEST_*                                   */
+    int32_t status; /* DP_OK, or the failure once state is DONE       */
+    uint64_t steps, step, cells, footprint_bytes;
+    /* seconds since the server was created; 0 until reached */
+    double submitted_s, admitted_s, started_s, completed_s;
+} dp_request_info;
+typedef struct dp_server_stats {
+    uint32_t queued, preparing, ready, running;
+    uint64_t used_bytes, budget_bytes, chunks;
+    double busy_s, uptime_s;
+} dp_server_stats;
+int dp_server_create(dp_engine *engine, const dp_server_config *config, dp_server **out);
+/* Stops the loop; every request still held is dropped. */
+void dp_server_destroy(dp_server *server);
+/* Copies the scene; DP_ERR_BUDGET if the request alone exceeds the budget. */
+int dp_server_submit(dp_server *server, const dp_scene *scene, uint64_t steps, uint64_t *ticket);
+int dp_server_query(dp_server *server, uint64_t ticket, dp_request_info *info);
+/* Blocks until the request is done; its failure status if it failed. */
+int dp_server_wait(dp_server *server, uint64_t ticket, uint32_t timeout_ms);
+/* The finished request's sim, borrowed until dp_server_release. */
+int dp_server_result(dp_server *server, uint64_t ticket, dp_sim **sim);
+/* Frees a finished request and its device memory (returns its budget). */
+int dp_server_release(dp_server *server, uint64_t ticket);
+int dp_server_statistics(dp_server *server, dp_server_stats *stats);
+
 #ifdef __cplusplus
 }
 #endif
