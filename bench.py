@@ -160,8 +160,8 @@ def bench_batched(args) -> int:
     for sim in sims:
         batch.add(sim, steps)
     if args.rows:
-        batch.set_tile(args.rows, 0)
-        tuned = {"rows": args.rows, "depth": 32, "ms_per_step": None}
+        batch.set_tile(args.rows, args.depth)
+        tuned = {"rows": args.rows, "depth": args.depth or 32, "ms_per_step": None}
     else:
         rows, depth, ms = batch.autotune(3)
         tuned = {"rows": rows, "depth": depth, "ms_per_step": ms}
@@ -340,7 +340,8 @@ def main() -> int:
     parser.add_argument("--steps", type=int, default=None, help="timed steps (default: the workload's)")
     parser.add_argument("--warmup", type=int, default=20)
     parser.add_argument("--workload", default="layered_768")
-    parser.add_argument("--rows", type=int, default=0, help="fused-kernel tile rows (0 = default)")
+    parser.add_argument("--rows", type=int, default=0, help="fused-kernel tile rows (0 = default / autotune)")
+    parser.add_argument("--depth", type=int, default=0, help="fused-kernel z-slab depth (0 = default)")
     parser.add_argument("--no-cpu", action="store_true", help="skip the CPU-oracle baseline")
     parser.add_argument("--cpu-seconds", type=float, default=10.0)
     parser.add_argument("--requests", type=int, default=200, help="served_stream: number of requests")
@@ -373,8 +374,8 @@ def main() -> int:
 
     context = headless_context()
     sim = Simulation(context, work.scene)
-    if args.rows:
-        sim.set_tile(args.rows)
+    if args.rows or args.depth:
+        sim.set_tile(args.rows, args.depth)
     cells = sim.cells
 
     # The step's access mix streamed with no stencil: the practical ceiling.

@@ -42,6 +42,9 @@ def main(argv: Optional[List[str]] = None) -> int:
     ap.add_argument("--calibrate-kernel", help="kernel of known traffic in the same runs")
     ap.add_argument("--calibrate-read-bytes", type=float, default=0.0)
     ap.add_argument("--calibrate-write-bytes", type=float, default=0.0)
+    ap.add_argument("--read-scale", type=float, default=0.0,
+                    help="FETCH_SIZE factor measured on another run (used without a calibration kernel)")
+    ap.add_argument("--write-scale", type=float, default=0.0)
     ap.add_argument("--cells", type=float, required=True)
     ap.add_argument("--bytes-per-cell", type=float, required=True)
     ap.add_argument("--peak-gbs", type=float, default=8000.0)
@@ -75,7 +78,8 @@ def main(argv: Optional[List[str]] = None) -> int:
              f"| algorithmic bytes / launch | {algorithmic / 1e9:.3f} GB ({args.bytes_per_cell:g} B × {args.cells:.0f} cells) |",
              f"| achieved (algorithmic / duration) | {achieved:.1f} GB/s = {100 * achieved / args.peak_gbs:.1f}% of {args.peak_gbs:.0f} GB/s |"]
 
-    read_scale = write_scale = 1.0
+    read_scale = args.read_scale or 1.0
+    write_scale = args.write_scale or 1.0
     if args.calibrate_kernel and args.fetch and args.calibrate_read_bytes:
         raw = counter_values(args.fetch, args.calibrate_kernel, "FETCH_SIZE")
         if raw:
@@ -102,6 +106,9 @@ def main(argv: Optional[List[str]] = None) -> int:
         summary["traffic_over_algorithmic"] = traffic / algorithmic
         lines.append(f"| HBM traffic / algorithmic | {traffic / algorithmic:.3f} |")
         lines.append(f"| HBM traffic rate | {traffic / (avg_ms / 1e3) / 1e9:.1f} GB/s |")
+    if args.read_scale or args.write_scale:
+        lines += ["", f"Counter scale factors ({read_scale:.3f} read, {write_scale:.3f} write) were measured "
+                  f"with the stream-probe kernel of known traffic on the 768³ run."]
     if args.calibrate_kernel:
         lines += ["", f"Counter calibration: `{args.calibrate_kernel}` moves "
                   f"{args.calibrate_read_bytes / 1e9:.3f} GB read and {args.calibrate_write_bytes / 1e9:.3f} GB "
