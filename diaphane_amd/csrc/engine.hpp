@@ -264,7 +264,8 @@ struct dp_sim {
                 a.intensity = accumulating ? intensity.as<double>() : nullptr;
                 if (kernel == DP_KERNEL_FUSED) {
                     fill_tiles(a, r, kAlignedX);
-                    launch_fused_aligned(a, r, (tile_flags & DP_TILE_NONTEMPORAL) != 0, stream);
+                    launch_fused_aligned(a, r, (tile_flags & DP_TILE_NONTEMPORAL) != 0,
+                                         (tile_flags & DP_TILE_BUFFER) != 0, stream);
                 } else {
                     fill_tiles(a, r, kOwnedX);
                     launch_fused(a, r, stream);

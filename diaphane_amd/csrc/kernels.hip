@@ -445,22 +445,34 @@ __global__ __launch_bounds__(kEnergyThreads) void k_energy(const float *e, const
 // The Yee step's HBM access mix with no stencil and no arithmetic: six f32
 // streams and one u8 stream read, six f32 streams written, 16 bytes per lane
 // per stream. Measures the bandwidth this access pattern can sustain.
+__device__ __forceinline__ void store_stream(float4 v, float4 *p) {
+    typedef float v4 __attribute__((ext_vector_type(4)));
+    const v4 x = {v.x, v.y, v.z, v.w};
+    __builtin_nontemporal_store(x, reinterpret_cast<v4 *>(p));
+}
+
+// One float4 of each of the six field streams (and 4 material bytes) per
+// thread, all loads issued before any store, stores non-temporal: the
+// fastest of the streaming forms measured on MI355X for this byte mix.
 __global__ __launch_bounds__(256) void k_stream_probe(const float4 *__restrict__ e, const float4 *__restrict__ h,
                                                       const uint32_t *__restrict__ material,
                                                       float4 *__restrict__ e_out, float4 *__restrict__ h_out,
                                                       uint64_t quads) {
-    const uint64_t stride = uint64_t(gridDim.x) * blockDim.x;
-    for (uint64_t q = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; q < quads; q += stride) {
-        const uint32_t m = material[q];
-        const float bias = float(m & 1u);
+    const uint64_t q = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (q >= quads) return;
+    const float bias = float(material[q] & 1u);
+    float4 a[3], b[3];
 #pragma unroll
-        for (int c = 0; c < 3; ++c) {
-            float4 a = e[c * quads + q], b = h[c * quads + q];
-            a.x += bias;
-            b.x += bias;
-            e_out[c * quads + q] = a;
-            h_out[c * quads + q] = b;
-        }
+    for (int c = 0; c < 3; ++c) {
+        a[c] = e[c * quads + q];
+        b[c] = h[c * quads + q];
+    }
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        a[c].x += bias;
+        b[c].x += bias;
+        store_stream(a[c], &e_out[c * quads + q]);
+        store_stream(b[c], &h_out[c * quads + q]);
     }
 }
 
@@ -469,7 +481,7 @@ __global__ __launch_bounds__(256) void k_stream_probe(const float4 *__restrict__
 void launch_stream_probe(const float *e, const float *h, const uint8_t *material, float *e_out, float *h_out,
                          uint64_t cells, hipStream_t stream) {
     const uint64_t quads = cells / 4;
-    const uint32_t blocks = uint32_t(std::min<uint64_t>((quads + 255) / 256, 256u * 64u));
+    const uint32_t blocks = uint32_t((quads + 255) / 256);
     k_stream_probe<<<blocks, 256, 0, stream>>>(reinterpret_cast<const float4 *>(e), reinterpret_cast<const float4 *>(h),
                                                reinterpret_cast<const uint32_t *>(material),
                                                reinterpret_cast<float4 *>(e_out), reinterpret_cast<float4 *>(h_out),
@@ -493,15 +505,15 @@ void launch_fused(const StepArgs &args, int rows, hipStream_t stream) {
     }
 }
 
-template <typename Offset>
+template <template <bool> class Access>
 static void launch_fused_aligned_as(const StepArgs &args, int rows, bool nontemporal, hipStream_t stream) {
     const uint32_t blocks = (args.tiles_total + 3) / 4;
-#define DPA_ROWS(R)                                                                \
-    case R:                                                                        \
-        if (nontemporal)                                                           \
-            k_fused_aligned<R, true, Offset><<<blocks, 256, 0, stream>>>(args);    \
-        else                                                                       \
-            k_fused_aligned<R, false, Offset><<<blocks, 256, 0, stream>>>(args);   \
+#define DPA_ROWS(R)                                                                   \
+    case R:                                                                           \
+        if (nontemporal)                                                              \
+            k_fused_aligned<R, Access<true>><<<blocks, 256, 0, stream>>>(args);       \
+        else                                                                          \
+            k_fused_aligned<R, Access<false>><<<blocks, 256, 0, stream>>>(args);      \
         break;
     switch (rows) {
         DPA_ROWS(4)
@@ -510,7 +522,7 @@ static void launch_fused_aligned_as(const StepArgs &args, int rows, bool nontemp
         DPA_ROWS(10)
         DPA_ROWS(12)
     default:
-        k_fused_aligned<4, true, Offset><<<blocks, 256, 0, stream>>>(args);
+        k_fused_aligned<4, Access<true>><<<blocks, 256, 0, stream>>>(args);
         break;
     }
 #undef DPA_ROWS
@@ -552,12 +564,21 @@ uint32_t batch_resident_blocks(int rows, bool nontemporal) {
     return uint32_t(per_cu) * uint32_t(prop.multiProcessorCount);
 }
 
-void launch_fused_aligned(const StepArgs &args, int rows, bool nontemporal, hipStream_t stream) {
+bool buffer_addressing_fits(uint64_t cells, uint32_t nx, uint32_t ny) {
+    // largest offset from a plane's base: 2 components + the next-plane step
+    // in the scalar offset, plus a lane's in-plane offset
+    const uint64_t comp = cells * sizeof(float), plane = uint64_t(nx) * ny * sizeof(float);
+    return 2 * comp + 2 * plane < 0xFFFFFFFFull;
+}
+
+void launch_fused_aligned(const StepArgs &args, int rows, bool nontemporal, bool buffer, hipStream_t stream) {
+    if (buffer && buffer_addressing_fits(args.cells, args.nx, args.ny))
+        launch_fused_aligned_as<BufferAccess>(args, rows, nontemporal, stream);
     // 32-bit byte offsets while one component fits in 4 GiB (1 Gi cells).
-    if (args.cells * sizeof(float) <= 0xFFFFFFFFull)
-        launch_fused_aligned_as<uint32_t>(args, rows, nontemporal, stream);
+    else if (args.cells * sizeof(float) <= 0xFFFFFFFFull)
+        launch_fused_aligned_as<Pointer32>(args, rows, nontemporal, stream);
     else
-        launch_fused_aligned_as<uint64_t>(args, rows, nontemporal, stream);
+        launch_fused_aligned_as<Pointer64>(args, rows, nontemporal, stream);
 }
 
 void launch_update_h(const StepArgs &args, int direction, hipStream_t stream) {

@@ -92,7 +92,10 @@ constexpr int kOwnedX = kLanes - 2;
 void launch_fused(const StepArgs &args, int rows, hipStream_t stream);
 // 64-aligned tiles; halo columns fetched by lanes 0 and 63.
 constexpr int kAlignedX = 64;
-void launch_fused_aligned(const StepArgs &args, int rows, bool nontemporal, hipStream_t stream);
+// `buffer`: address fields through buffer resources (32-bit lane offsets,
+// component offsets in the scalar offset) when the grid allows it.
+void launch_fused_aligned(const StepArgs &args, int rows, bool nontemporal, bool buffer, hipStream_t stream);
+bool buffer_addressing_fits(uint64_t cells, uint32_t nx, uint32_t ny);
 // One launch advancing every active instance of a batch by one step.
 // `blocks` is the persistent grid size (workgroups of 4 waves).
 void launch_fused_batch(const InstanceDesc *instances, const StepSlot *slots, const uint32_t *owners,

@@ -22,6 +22,7 @@ KERNEL_TWO_PASS = 1
 KERNEL_FUSED_OVERLAP = 2
 TILE_XCD_REMAP = 1
 TILE_NONTEMPORAL = 2
+TILE_BUFFER = 4
 
 BUFFER_ELECTRIC = 0
 BUFFER_MAGNETIC = 1

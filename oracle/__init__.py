@@ -70,6 +70,11 @@ def lib() -> ctypes.CDLL:
             "or_sim_intensity": (POINTER(c_float), [sim]),
             "or_sim_intensity_f64": (POINTER(c_double), [sim]),
             "or_sim_restore": (c_int, [sim, POINTER(c_float), POINTER(c_float), c_size_t, c_uint64]),
+            "or_sim_f64_begin": (c_int, [sim]),
+            "or_sim_f64_advance": (c_int, [sim, c_uint64]),
+            "or_sim_electric_f64": (POINTER(c_double), [sim]),
+            "or_sim_magnetic_f64": (POINTER(c_double), [sim]),
+            "or_sim_step_count_f64": (c_uint64, [sim]),
             "or_sim_energy": (None, [sim, POINTER(c_double), POINTER(c_double)]),
             "or_sim_is_finite": (c_int, [sim]),
         }
@@ -279,6 +284,19 @@ class Simulation:
 
     def intensity(self) -> np.ndarray:
         return np.ctypeslib.as_array(lib().or_sim_intensity(self._handle), shape=(self.cells,))
+
+    # -- the f64 track (the same update in double, for error statements) --
+    def f64_begin(self) -> None:
+        _check(lib().or_sim_f64_begin(self._handle))
+
+    def f64_advance_by(self, steps: int) -> None:
+        _check(lib().or_sim_f64_advance(self._handle, steps))
+
+    def electric_f64(self) -> np.ndarray:
+        return np.ctypeslib.as_array(lib().or_sim_electric_f64(self._handle), shape=(3 * self.cells,))
+
+    def magnetic_f64(self) -> np.ndarray:
+        return np.ctypeslib.as_array(lib().or_sim_magnetic_f64(self._handle), shape=(3 * self.cells,))
 
     def intensity_f64(self) -> np.ndarray:
         return np.ctypeslib.as_array(lib().or_sim_intensity_f64(self._handle), shape=(self.cells,))

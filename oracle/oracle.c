@@ -772,6 +772,9 @@ struct or_sim {
     uint32_t source_count;
     uint64_t step;
     int threads;
+    /* the f64 track: the same update on double fields (or_sim_f64_*) */
+    double *electric64, *magnetic64;
+    uint64_t step64;
 };
 
 or_sim *or_sim_create(const dp_scene *scene) {
@@ -845,6 +848,8 @@ void or_sim_destroy(or_sim *s) {
     absorber_free(&s->absorber);
     free(s->intensity);
     free(s->intensity64);
+    free(s->electric64);
+    free(s->magnetic64);
     free(s->sources);
     free(s);
 }
@@ -1177,4 +1182,126 @@ float or_grid_worst_ratio(const dp_grid_spec *spec) {
     for (int a = 0; a < 3; ++a) w = fmaxf(w, spacing_worst_ratio(&g.spacing[a]));
     grid_free(&g);
     return w;
+}
+
+/* ---------------------------------------------------------------------- */
+/* The f64 track: the forward update of src/cpu.rs:174-198 with every      */
+/* field value and every arithmetic operation in double. The inputs are    */
+/* the same f32 tables (gains, absorber, coefficients, injection values   */
+/* and weights), promoted. It measures the f32 rounding error of the      */
+/* update itself; it is not bit-compared with anything.                   */
+/* ---------------------------------------------------------------------- */
+int or_sim_f64_begin(or_sim *s) {
+    free(s->electric64);
+    free(s->magnetic64);
+    s->electric64 = malloc(sizeof(double) * 3 * s->cells);
+    s->magnetic64 = malloc(sizeof(double) * 3 * s->cells);
+    if (!s->electric64 || !s->magnetic64) return fail("out of host memory for the f64 track");
+    for (uint64_t i = 0; i < 3 * s->cells; ++i) {
+        s->electric64[i] = (double)s->electric[i];
+        s->magnetic64[i] = (double)s->magnetic[i];
+    }
+    s->step64 = s->step;
+    return 0;
+}
+
+double *or_sim_electric_f64(or_sim *s) { return s->electric64; }
+double *or_sim_magnetic_f64(or_sim *s) { return s->magnetic64; }
+uint64_t or_sim_step_count_f64(const or_sim *s) { return s->step64; }
+
+static void sweep_magnetic64(or_sim *s) {
+    const uint32_t *extent = s->grid.extent;
+    const uint64_t strides[3] = {1, extent[0], (uint64_t)extent[0] * extent[1]};
+    int threads = thread_count(s);
+    for (int a = 0; a < 3; ++a) {
+        int b = (a + 1) % 3, c = (a + 2) % 3;
+        const float *gain_b = s->magnetic_gains[b], *gain_c = s->magnetic_gains[c];
+        double *target = s->magnetic64 + (uint64_t)a * s->cells;
+        const double *field_b = s->electric64 + (uint64_t)b * s->cells;
+        const double *field_c = s->electric64 + (uint64_t)c * s->cells;
+#pragma omp parallel for schedule(static) num_threads(threads)
+        for (int64_t zz = 0; zz < (int64_t)extent[2]; ++zz) {
+            uint32_t z = (uint32_t)zz;
+            for (uint32_t y = 0; y < extent[1]; ++y)
+                for (uint32_t x = 0; x < extent[0]; ++x) {
+                    uint32_t coord[3] = {x, y, z};
+                    uint64_t i = ((uint64_t)z * extent[1] + y) * extent[0] + x;
+                    /* past a high wall the neighbour reads as zero (src/cpu.rs:335-374) */
+                    double next_c = coord[b] + 1 < extent[b] ? field_c[i + strides[b]] : 0.0;
+                    double next_b = coord[c] + 1 < extent[c] ? field_b[i + strides[c]] : 0.0;
+                    double curl = (double)gain_b[coord[b]] * (next_c - field_c[i]) -
+                                  (double)gain_c[coord[c]] * (next_b - field_b[i]);
+                    float absorb_row;
+                    const float *absorb_x = magnetic_row(&s->absorber, a, y, z, &absorb_row);
+                    coeff_t k = s->coefficients[s->material_index[i]];
+                    double loss = (double)k.magnetic_loss + (double)absorb_row + (double)absorb_x[x];
+                    target[i] = ((1.0 - loss) * target[i] - (double)k.magnetic_gain * curl) / (1.0 + loss);
+                }
+        }
+    }
+}
+
+static void sweep_electric64(or_sim *s) {
+    const uint32_t *extent = s->grid.extent;
+    const uint64_t strides[3] = {1, extent[0], (uint64_t)extent[0] * extent[1]};
+    int threads = thread_count(s);
+    for (int a = 0; a < 3; ++a) {
+        int b = (a + 1) % 3, c = (a + 2) % 3;
+        const float *gain_b = s->electric_gains[b], *gain_c = s->electric_gains[c];
+        double *target = s->electric64 + (uint64_t)a * s->cells;
+        const double *field_b = s->magnetic64 + (uint64_t)b * s->cells;
+        const double *field_c = s->magnetic64 + (uint64_t)c * s->cells;
+#pragma omp parallel for schedule(static) num_threads(threads)
+        for (int64_t zz = 0; zz < (int64_t)extent[2]; ++zz) {
+            uint32_t z = (uint32_t)zz;
+            for (uint32_t y = 0; y < extent[1]; ++y)
+                for (uint32_t x = 0; x < extent[0]; ++x) {
+                    uint32_t coord[3] = {x, y, z};
+                    if (coord[b] == 0 || coord[c] == 0) continue; /* low-wall PEC */
+                    uint64_t i = ((uint64_t)z * extent[1] + y) * extent[0] + x;
+                    double curl = (double)gain_b[coord[b]] * (field_c[i] - field_c[i - strides[b]]) -
+                                  (double)gain_c[coord[c]] * (field_b[i] - field_b[i - strides[c]]);
+                    float absorb_row;
+                    const float *absorb_x = electric_row(&s->absorber, a, y, z, &absorb_row);
+                    coeff_t k = s->coefficients[s->material_index[i]];
+                    double loss = (double)k.electric_loss + (double)absorb_row + (double)absorb_x[x];
+                    target[i] = ((1.0 - loss) * target[i] + (double)k.electric_gain * curl) / (1.0 + loss);
+                }
+        }
+    }
+}
+
+static void apply_sources64(or_sim *s, float time, int magnetic) {
+    const uint32_t *extent = s->grid.extent;
+    for (uint32_t k = 0; k < s->source_count; ++k) {
+        dp_injection inj[2];
+        source_injections(&s->sources[k], &s->grid, time, inj);
+        for (int j = 0; j < 2; ++j) {
+            const dp_injection *q = &inj[j];
+            if (q->value == 0.0f || (int)q->magnetic != magnetic) continue;
+            double *target = (q->magnetic ? s->magnetic64 : s->electric64) + (uint64_t)q->component * s->cells;
+            for (uint32_t dz = 0; dz < q->extent[2]; ++dz)
+                for (uint32_t dy = 0; dy < q->extent[1]; ++dy)
+                    for (uint32_t dx = 0; dx < q->extent[0]; ++dx) {
+                        uint32_t c[3] = {q->origin[0] + dx, q->origin[1] + dy, q->origin[2] + dz};
+                        float p[3];
+                        grid_cell_center(&s->grid, c, p);
+                        uint64_t i = ((uint64_t)c[2] * extent[1] + c[1]) * extent[0] + c[0];
+                        target[i] += (double)q->value * (double)injection_weight(q, p);
+                    }
+        }
+    }
+}
+
+int or_sim_f64_advance(or_sim *s, uint64_t steps) {
+    if (!s->electric64) return fail("the f64 track was not started");
+    for (uint64_t n = 0; n < steps; ++n) {
+        float time = (float)(s->step64 + 1) * grid_time_step(&s->grid);
+        sweep_magnetic64(s);
+        apply_sources64(s, time, 1);
+        sweep_electric64(s);
+        apply_sources64(s, time, 0);
+        s->step64 += 1;
+    }
+    return 0;
 }

@@ -100,6 +100,16 @@ int or_sim_restore(or_sim *sim, const float *electric, const float *magnetic,
 void or_sim_energy(const or_sim *sim, double *electric, double *magnetic);
 int or_sim_is_finite(const or_sim *sim);
 
+/* The f64 track: the forward update with double fields and arithmetic on
+ * the same (promoted) f32 tables and injections. begin copies the current
+ * f32 state; advance moves only the f64 fields. For stating the f32
+ * rounding error of the update per workload. */
+int or_sim_f64_begin(or_sim *sim);
+int or_sim_f64_advance(or_sim *sim, uint64_t steps);
+double *or_sim_electric_f64(or_sim *sim);
+double *or_sim_magnetic_f64(or_sim *sim);
+uint64_t or_sim_step_count_f64(const or_sim *sim);
+
 #ifdef __cplusplus
 }
 #endif

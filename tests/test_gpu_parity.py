@@ -245,6 +245,8 @@ def test_fused_tiles_agree_bit_for_bit(context):
         for rows in (4, 6, 8, 10, 12):
             variants.append((kernel, rows, 0, 1))
     variants += [(KERNEL_FUSED, 6, 7, 3), (KERNEL_FUSED, 4, 13, 2), (KERNEL_FUSED, 8, 40, 0)]
+    # buffer-resource addressing (flag 4), with and without non-temporal stores
+    variants += [(KERNEL_FUSED, 4, 0, 6), (KERNEL_FUSED, 6, 7, 7), (KERNEL_FUSED, 12, 40, 4), (KERNEL_FUSED, 8, 3, 5)]
     for kernel, rows, depth, flags in variants:
         sim = Simulation(context, scene)
         sim.set_kernel(kernel)
