@@ -342,6 +342,9 @@ def main() -> int:
     parser.add_argument("--workload", default="layered_768")
     parser.add_argument("--rows", type=int, default=0, help="fused-kernel tile rows (0 = default / autotune)")
     parser.add_argument("--depth", type=int, default=0, help="fused-kernel z-slab depth (0 = default)")
+    parser.add_argument("--tile-flags", type=int, default=-1,
+                        help="single-solve tile flags (1 XCD remap, 2 non-temporal stores, 4 buffer addressing; "
+                             "-1 = default)")
     parser.add_argument("--no-cpu", action="store_true", help="skip the CPU-oracle baseline")
     parser.add_argument("--cpu-seconds", type=float, default=10.0)
     parser.add_argument("--requests", type=int, default=200, help="served_stream: number of requests")
@@ -374,8 +377,8 @@ def main() -> int:
 
     context = headless_context()
     sim = Simulation(context, work.scene)
-    if args.rows or args.depth:
-        sim.set_tile(args.rows, args.depth)
+    if args.rows or args.depth or args.tile_flags >= 0:
+        sim.set_tile(args.rows, args.depth, args.tile_flags)
     cells = sim.cells
 
     # The step's access mix streamed with no stencil: the practical ceiling.

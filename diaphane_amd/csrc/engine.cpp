@@ -423,7 +423,8 @@ int dp_sim_set_tile(dp_sim *sim, int rows, int depth, int flags) {
         if (depth < 0) fail(DP_ERR_INVALID, "depth must be non-negative");
         sim->rows = rows;
         sim->depth = static_cast<uint32_t>(depth);
-        if (flags >= 0) sim->tile_flags = flags & (DP_TILE_XCD_REMAP | DP_TILE_NONTEMPORAL | DP_TILE_BUFFER);
+        if (flags >= 0)
+            sim->tile_flags = flags & (DP_TILE_XCD_REMAP | DP_TILE_NONTEMPORAL | DP_TILE_BUFFER | 0xFF00);
     });
 }
 

@@ -243,7 +243,12 @@ struct dp_sim {
         a.depth = std::min<uint32_t>(nz, depth ? depth : 32);
         a.tiles_z = (nz + a.depth - 1) / a.depth;
         a.tiles_total = a.tiles_x * a.tiles_y * a.tiles_z;
-        a.xcd_remap = (tile_flags & DP_TILE_XCD_REMAP) ? 1 : 0;
+        if (tile_flags & DP_TILE_XCD_REMAP) {
+            const uint32_t group = (uint32_t(tile_flags) >> 8) & 0xFFu;
+            a.xcd_remap = group ? group : 8;
+        } else {
+            a.xcd_remap = 0;
+        }
     }
 
     void advance(uint64_t steps) {

@@ -63,13 +63,17 @@ __device__ __forceinline__ float intensity_term(float ex, float ey, float ez) {
     return sum;
 }
 
-// Workgroups are dispatched to the 8 XCDs round-robin; renumber them so each
-// XCD walks a contiguous run of tiles and neighbouring tiles share its L2.
-__device__ __forceinline__ uint32_t xcd_contiguous(uint32_t block, uint32_t blocks) {
+// Workgroups are dispatched to the 8 XCDs round-robin. Renumber them so each
+// XCD takes runs of `group` consecutive blocks while the eight XCDs
+// still sweep the grid together: within a run, a tile's neighbours along y
+// sit on the same XCD and share its L2. A bijection on [0, blocks); the tail
+// past the last whole group of 8·group blocks keeps its numbering.
+__device__ __forceinline__ uint32_t xcd_grouped(uint32_t block, uint32_t blocks, uint32_t group) {
     constexpr uint32_t kXcds = 8;
-    const uint32_t xcd = block % kXcds, local = block / kXcds;
-    const uint32_t per = blocks / kXcds, extra = blocks % kXcds;
-    return xcd < extra ? xcd * (per + 1) + local : extra * (per + 1) + (xcd - extra) * per + local;
+    const uint32_t span = kXcds * group;
+    if (block >= blocks / span * span) return block;
+    const uint32_t g = block / span, r = block % span;
+    return g * span + (r % kXcds) * group + r / kXcds;
 }
 
 // ---------------------------------------------------------------------------
@@ -99,7 +103,7 @@ __global__ __launch_bounds__(256) void k_fused(StepArgs p) {
     __syncthreads();
 
     uint32_t block = blockIdx.x;
-    if (p.xcd_remap) block = xcd_contiguous(block, gridDim.x);
+    if (p.xcd_remap) block = xcd_grouped(block, gridDim.x, p.xcd_remap);
     const uint32_t tile = __builtin_amdgcn_readfirstlane(block * (blockDim.x / kLanes) + threadIdx.x / kLanes);
     if (tile >= p.tiles_total) return;
     const int lane = threadIdx.x & (kLanes - 1);

@@ -54,7 +54,7 @@ struct StepArgs {
     // fused-kernel tiling
     uint32_t tiles_x, tiles_y, tiles_z, depth;
     uint32_t tiles_total;
-    uint32_t xcd_remap;
+    uint32_t xcd_remap;  // 0, or the run length in blocks per XCD
 };
 
 // One grid of a batch, as the batch kernel sees it (device memory).

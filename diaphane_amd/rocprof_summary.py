@@ -51,6 +51,7 @@ def main(argv: Optional[List[str]] = None) -> int:
     ap.add_argument("--command", default="")
     ap.add_argument("--out", required=True)
     ap.add_argument("--json")
+    ap.add_argument("--note", action="append", default=[], help="a line appended to the summary")
     args = ap.parse_args(argv)
 
     stats = [r for r in _rows(args.trace) if _match(r["Name"], args.kernel)]
@@ -100,12 +101,12 @@ def main(argv: Optional[List[str]] = None) -> int:
         traffic += per_launch
         summary[f"{label}_bytes_per_launch"] = per_launch
         summary[f"{label}_scale"] = scale
-        lines.append(f"| HBM {label} / launch ({counter} × {scale:.3f}) | {per_launch / 1e9:.3f} GB |")
+        lines.append(f"| L2→memory {label} / launch ({counter} × {scale:.3f}) | {per_launch / 1e9:.3f} GB |")
     if traffic:
         summary["traffic_bytes_per_launch"] = traffic
         summary["traffic_over_algorithmic"] = traffic / algorithmic
-        lines.append(f"| HBM traffic / algorithmic | {traffic / algorithmic:.3f} |")
-        lines.append(f"| HBM traffic rate | {traffic / (avg_ms / 1e3) / 1e9:.1f} GB/s |")
+        lines.append(f"| L2→memory traffic / algorithmic | {traffic / algorithmic:.3f} |")
+        lines.append(f"| L2→memory traffic rate | {traffic / (avg_ms / 1e3) / 1e9:.1f} GB/s |")
     if args.read_scale or args.write_scale:
         lines += ["", f"Counter scale factors ({read_scale:.3f} read, {write_scale:.3f} write) were measured "
                   f"with the stream-probe kernel of known traffic on the 768³ run."]
@@ -113,6 +114,8 @@ def main(argv: Optional[List[str]] = None) -> int:
         lines += ["", f"Counter calibration: `{args.calibrate_kernel}` moves "
                   f"{args.calibrate_read_bytes / 1e9:.3f} GB read and {args.calibrate_write_bytes / 1e9:.3f} GB "
                   f"written per launch; the counters' ratio to that gives the scale factors above."]
+    if args.note:
+        lines += [""] + args.note
     with open(args.out, "w") as handle:
         handle.write("\n".join(lines) + "\n")
     if args.json:

@@ -278,11 +278,16 @@ enum { DP_KERNEL_FUSED = 0, DP_KERNEL_TWO_PASS = 1, DP_KERNEL_FUSED_OVERLAP = 2 
 enum { DP_TILE_XCD_REMAP = 1, DP_TILE_NONTEMPORAL = 2, DP_TILE_BUFFER = 4 };
 int dp_sim_set_kernel(dp_sim *sim, int kernel);
 /* Fused-kernel tiling: rows per wave along y (4, 6, 8, 10 or 12), z-slab
- * depth in planes, and flags: DP_TILE_XCD_REMAP renumbers workgroups so each
- * XCD walks a contiguous run of tiles, DP_TILE_NONTEMPORAL streams the field
- * stores past the caches, DP_TILE_BUFFER addresses the fields through buffer
- * resources with 32-bit lane offsets (used when the grid's offsets fit). 0 for rows/depth selects the tuned default;
- * flags < 0 keeps the current flags. Results never depend on any of it. */
+ * depth in planes, and flags:
+ *   DP_TILE_XCD_REMAP     renumbers workgroups so each XCD takes runs of G
+ *                         consecutive blocks of tiles (G in bits 8..15 of
+ *                         flags; 0 means 8);
+ *   DP_TILE_NONTEMPORAL   streams the field stores past the caches;
+ *   DP_TILE_BUFFER        addresses the fields through buffer resources with
+ *                         32-bit lane offsets (used when the grid's offsets
+ *                         fit).
+ * 0 for rows/depth selects the tuned default; flags < 0 keeps the current
+ * flags. Results never depend on any of it. */
 int dp_sim_set_tile(dp_sim *sim, int rows, int depth, int flags);
 
 /* HIP-event timing on the sim's stream: record event `slot` (0..7) now;
