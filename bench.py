@@ -164,7 +164,7 @@ def bench_batched(args) -> int:
         tuned = {"rows": args.rows, "depth": args.depth or 32, "ms_per_step": None}
     else:
         rows, depth, ms = batch.autotune(3)
-        tuned = {"rows": rows, "depth": depth, "ms_per_step": ms}
+        tuned = dict(batch.tiling(), ms_per_step=ms)
     log(f"tiling {tuned}")
     batch.wait()
     t0 = time.perf_counter()

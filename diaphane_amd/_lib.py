@@ -105,6 +105,7 @@ def _declare(lib: ctypes.CDLL) -> None:
         "dp_batch_set_graph_steps": (c_int, [c_void_p, c_uint32]),
         "dp_batch_set_tile": (c_int, [c_void_p, c_int, c_int, c_int]),
         "dp_batch_autotune": (c_int, [c_void_p, c_uint32, POINTER(c_int), POINTER(c_int), POINTER(c_float)]),
+        "dp_batch_tiling": (c_int, [c_void_p, POINTER(c_int), POINTER(c_int), POINTER(c_int)]),
         "dp_batch_size": (c_uint32, [c_void_p]),
         "dp_batch_cells": (c_uint64, [c_void_p]),
         "dp_batch_launch_count": (c_uint64, [c_void_p]),

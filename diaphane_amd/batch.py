@@ -13,6 +13,8 @@ from ctypes import byref, c_float, c_int, c_void_p
 from ._lib import check, lib
 from .gpu import Context, Simulation
 
+WAVE_PER_TILE = 16  # one wave per tile instead of the persistent grid
+
 
 class Batch:
     """A set of simulations advanced together on one stream."""
@@ -70,6 +72,12 @@ class Batch:
         rows, depth, ms = c_int(), c_int(), c_float()
         check(lib().dp_batch_autotune(self._handle, trials, byref(rows), byref(depth), byref(ms)))
         return rows.value, depth.value, ms.value
+
+    def tiling(self) -> dict:
+        rows, depth, flags = c_int(), c_int(), c_int()
+        check(lib().dp_batch_tiling(self._handle, byref(rows), byref(depth), byref(flags)))
+        return {"rows": rows.value, "depth": depth.value, "wave_per_tile": bool(flags.value & WAVE_PER_TILE),
+                "nontemporal": bool(flags.value & 2)}
 
     def __len__(self) -> int:
         return int(lib().dp_batch_size(self._handle))

@@ -534,21 +534,34 @@ static void launch_fused_aligned_as(const StepArgs &args, int rows, bool nontemp
 
 template <int R>
 static void launch_batch_rows(const InstanceDesc *instances, const StepSlot *slots, const uint32_t *owners,
-                              uint32_t n_tiles, const InjDesc *injections, bool nontemporal, uint32_t blocks,
-                              hipStream_t stream) {
-    if (nontemporal)
+                              uint32_t n_tiles, const InjDesc *injections, bool nontemporal, bool direct,
+                              uint32_t blocks, hipStream_t stream) {
+    if (direct) {
+        const uint32_t grid = (n_tiles + 3) / 4;
+        if (nontemporal)
+            k_fused_batch_direct<R, true><<<grid, 256, 0, stream>>>(instances, slots, owners, n_tiles, injections);
+        else
+            k_fused_batch_direct<R, false><<<grid, 256, 0, stream>>>(instances, slots, owners, n_tiles, injections);
+    } else if (nontemporal) {
         k_fused_batch<R, true><<<blocks, 256, 0, stream>>>(instances, slots, owners, n_tiles, injections);
-    else
+    } else {
         k_fused_batch<R, false><<<blocks, 256, 0, stream>>>(instances, slots, owners, n_tiles, injections);
+    }
 }
 
 void launch_fused_batch(const InstanceDesc *instances, const StepSlot *slots, const uint32_t *owners,
-                        uint32_t n_tiles, const InjDesc *injections, int rows, bool nontemporal, uint32_t blocks,
-                        hipStream_t stream) {
+                        uint32_t n_tiles, const InjDesc *injections, int rows, bool nontemporal, bool direct,
+                        uint32_t blocks, hipStream_t stream) {
     switch (rows) {
-    case 6: launch_batch_rows<6>(instances, slots, owners, n_tiles, injections, nontemporal, blocks, stream); break;
-    case 8: launch_batch_rows<8>(instances, slots, owners, n_tiles, injections, nontemporal, blocks, stream); break;
-    default: launch_batch_rows<4>(instances, slots, owners, n_tiles, injections, nontemporal, blocks, stream); break;
+    case 6:
+        launch_batch_rows<6>(instances, slots, owners, n_tiles, injections, nontemporal, direct, blocks, stream);
+        break;
+    case 8:
+        launch_batch_rows<8>(instances, slots, owners, n_tiles, injections, nontemporal, direct, blocks, stream);
+        break;
+    default:
+        launch_batch_rows<4>(instances, slots, owners, n_tiles, injections, nontemporal, direct, blocks, stream);
+        break;
     }
 }
 

@@ -63,12 +63,13 @@ def mixed_scenes():
     return instances, [photon, plane, cavity]
 
 
-@pytest.mark.parametrize("graph_steps", [16, 0])
-def test_batched_members_match_solo_runs_and_the_oracle(context, graph_steps):
+@pytest.mark.parametrize("graph_steps,flags", [(16, 2), (0, 2), (16, 18)])
+def test_batched_members_match_solo_runs_and_the_oracle(context, graph_steps, flags):
     instances, scenes = mixed_scenes()
     steps = 40  # two graphs of 16 and a tail of 8 when graph_steps = 16
     batch = Batch(context)
     batch.set_graph_steps(graph_steps)
+    batch.set_tile(0, 0, flags)
     members = [seeded(context, i) for i in instances] + [Simulation(context, s) for s in scenes]
     for sim in members:
         batch.add(sim)
@@ -174,11 +175,12 @@ def test_a_member_refuses_solo_advance_and_leaves_when_destroyed(context):
     batch.wait()
 
 
-@pytest.mark.parametrize("rows,depth", [(6, 16), (8, 5), (4, 64)])
-def test_tile_shapes_do_not_change_results(context, rows, depth):
+@pytest.mark.parametrize("rows,depth,flags", [(6, 16, 2), (8, 5, 0), (4, 64, 2), (4, 32, 18), (6, 7, 16)])
+def test_tile_shapes_do_not_change_results(context, rows, depth, flags):
+    # flags 16: one wave per tile instead of the persistent grid
     instances = batched_uneven(3, seed=14, lo=22, hi=80)
     batch = Batch(context)
-    batch.set_tile(rows, depth)
+    batch.set_tile(rows, depth, flags)
     members = [seeded(context, i) for i in instances]
     for sim in members:
         batch.add(sim)
