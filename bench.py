@@ -383,6 +383,15 @@ def main() -> int:
 
     # The step's access mix streamed with no stencil: the practical ceiling.
     probe_gbs = sim.stream_probe(5)
+    # Tile shape by the engine's autotuner unless given (the fields are unchanged).
+    if args.rows or args.depth or args.tile_flags >= 0:
+        tiling = {"rows": args.rows or 4, "depth": args.depth or 32, "flags": args.tile_flags, "autotuned": False}
+    else:
+        tiling = dict(sim.autotune(3), autotuned=True)
+        if accumulate_from is not None and accumulate_from < steps:
+            sim.set_accumulate_intensity(True)  # tune the accumulating steps' shape too
+            tiling["with_intensity"] = sim.autotune(3)
+            sim.set_accumulate_intensity(False)
 
     if args.warmup:
         sim.advance_by(args.warmup)
@@ -441,6 +450,7 @@ def main() -> int:
             "cells": cells,
             "extent": list(sim.extent),
             "steps_with_intensity": phase_b,
+            "tiling": tiling,
             "note": work.note,
         },
         "roofline": {

@@ -91,6 +91,7 @@ def _declare(lib: ctypes.CDLL) -> None:
         "dp_sim_device_ptr": (c_int, [sim, c_int, POINTER(c_void_p)]),
         "dp_sim_set_kernel": (c_int, [sim, c_int]),
         "dp_sim_set_tile": (c_int, [sim, c_int, c_int, c_int]),
+        "dp_sim_autotune": (c_int, [sim, c_uint32, POINTER(c_int), POINTER(c_int), POINTER(c_int), POINTER(c_float)]),
         "dp_sim_record_event": (c_int, [sim, c_int]),
         "dp_sim_elapsed_ms": (c_int, [sim, c_int, c_int, POINTER(c_float)]),
         "dp_sim_launch_count": (c_uint64, [sim]),

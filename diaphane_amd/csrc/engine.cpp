@@ -3,6 +3,7 @@
 #include "engine.hpp"
 
 #include <chrono>
+#include <limits>
 #include <thread>
 
 #define DPA_VERSION "diaphane_amd 0.1.0 (gfx950)"
@@ -421,10 +422,80 @@ int dp_sim_set_tile(dp_sim *sim, int rows, int depth, int flags) {
         if (std::find(std::begin(allowed), std::end(allowed), rows) == std::end(allowed))
             fail(DP_ERR_INVALID, "rows must be one of 4, 6, 8, 10, 12 (or 0 for the default)");
         if (depth < 0) fail(DP_ERR_INVALID, "depth must be non-negative");
-        sim->rows = rows;
-        sim->depth = static_cast<uint32_t>(depth);
-        if (flags >= 0)
-            sim->tile_flags = flags & (DP_TILE_XCD_REMAP | DP_TILE_NONTEMPORAL | DP_TILE_BUFFER | 0xFF00);
+        for (auto &t : sim->tiling) {
+            t.rows = rows;
+            t.depth = static_cast<uint32_t>(depth);
+            if (flags >= 0) t.flags = flags & (DP_TILE_XCD_REMAP | DP_TILE_NONTEMPORAL | DP_TILE_BUFFER | 0xFF00);
+        }
+    });
+}
+
+int dp_sim_autotune(dp_sim *sim, uint32_t trials, int *rows, int *depth, int *flags, float *ms_per_step) {
+    return guarded([&] {
+        if (!sim) fail(DP_ERR_INVALID, "null sim");
+        if (sim->batch) fail(DP_ERR_INVALID, "this sim is advanced by its batch; tune the batch instead");
+        if (sim->kernel != DP_KERNEL_FUSED) fail(DP_ERR_UNSUPPORTED, "the autotuner tunes the aligned fused kernel");
+        if (trials == 0 || trials > 1000) fail(DP_ERR_INVALID, "trials must be in 1..1000");
+        static const int row_choices[] = {4, 6, 8};
+        static const uint32_t depth_choices[] = {16, 32, 64};
+        const bool buffer_fits = buffer_addressing_fits(sim->cells, sim->grid.extent[0], sim->grid.extent[1]);
+        const int flag_choices[] = {DP_TILE_NONTEMPORAL, DP_TILE_NONTEMPORAL | DP_TILE_BUFFER};
+        // Trial launches read the current half of each ping-pong pair and
+        // write the idle half, with no injections; a sim that accumulates
+        // intensity is timed with the read-modify-write going to a scratch
+        // buffer. The state is unchanged.
+        StepArgs a = sim->args(sim->current, 1 - sim->current);
+        a.injections = sim->injections.as<InjDesc>();
+        a.n_magnetic = a.n_electric = 0;
+        DeviceBuffer scratch;
+        a.intensity = nullptr;
+        if (sim->accumulating) {
+            scratch.allocate(sim->cells * sizeof(double));
+            check(hipMemsetAsync(scratch.ptr, 0, sim->cells * sizeof(double), sim->stream), "hipMemsetAsync");
+            a.intensity = scratch.as<double>();
+        }
+        dp_sim::Tiling trial;
+        float best = std::numeric_limits<float>::infinity();
+        int best_rows = 4, best_flags = DP_TILE_NONTEMPORAL;
+        uint32_t best_depth = 32;
+        for (int f : flag_choices) {
+            if ((f & DP_TILE_BUFFER) && !buffer_fits) continue;
+            for (int r : row_choices)
+                for (uint32_t d : depth_choices) {
+                    trial.rows = r;
+                    trial.depth = d;
+                    trial.flags = f;
+                    sim->fill_tiles(a, r, kAlignedX, trial);
+                    auto launch = [&] {
+                        launch_fused_aligned(a, r, (f & DP_TILE_NONTEMPORAL) != 0, (f & DP_TILE_BUFFER) != 0,
+                                             sim->stream);
+                    };
+                    launch();  // the first launch of a shape loads its code object
+                    check(hipGetLastError(), "kernel launch");
+                    check(hipEventRecord(sim->events[6], sim->stream), "hipEventRecord");
+                    for (uint32_t t = 0; t < trials; ++t) launch();
+                    check(hipEventRecord(sim->events[7], sim->stream), "hipEventRecord");
+                    check(hipEventSynchronize(sim->events[7]), "hipEventSynchronize");
+                    float ms = 0.0f;
+                    check(hipEventElapsedTime(&ms, sim->events[6], sim->events[7]), "hipEventElapsedTime");
+                    ms /= static_cast<float>(trials);
+                    if (ms < best) {
+                        best = ms;
+                        best_rows = r;
+                        best_depth = d;
+                        best_flags = f;
+                    }
+                }
+        }
+        check(hipStreamSynchronize(sim->stream), "hipStreamSynchronize");  // before the scratch goes
+        dp_sim::Tiling &tuned = sim->tiling[sim->accumulating ? 1 : 0];
+        tuned.rows = best_rows;
+        tuned.depth = best_depth;
+        tuned.flags = best_flags;
+        if (rows) *rows = best_rows;
+        if (depth) *depth = static_cast<int>(best_depth);
+        if (flags) *flags = best_flags;
+        if (ms_per_step) *ms_per_step = best;
     });
 }
 

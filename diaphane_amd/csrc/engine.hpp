@@ -123,11 +123,17 @@ struct dp_sim {
     bool accumulating = false;
     uint64_t accumulated = 0;
     int kernel = DP_KERNEL_FUSED;
-    int rows = 0;  // 0: tuned default
-    uint32_t depth = 0;  // 0: tuned default
-    // Measured on MI355X at 768³: non-temporal stores help (~2%); renumbering
-    // workgroups per XCD does not (the halos already hit L2).
-    int tile_flags = DP_TILE_NONTEMPORAL;
+    // Fused-kernel tiling for plain steps [0] and for steps that accumulate
+    // intensity [1], whose extra read-modify-write can favour another shape.
+    // rows/depth 0 select the defaults. Measured on MI355X at 768³:
+    // non-temporal stores help (~2%); renumbering workgroups per XCD does not.
+    struct Tiling {
+        int rows = 0;
+        uint32_t depth = 0;
+        int flags = DP_TILE_NONTEMPORAL;
+    };
+    Tiling tiling[2];
+    const Tiling &active() const { return tiling[accumulating ? 1 : 0]; }
     uint64_t launches = 0;
     hipEvent_t events[8] = {};
 
@@ -167,7 +173,7 @@ struct dp_sim {
     }
 
     int effective_rows() const {
-        if (rows > 0) return rows;
+        if (active().rows > 0) return active().rows;
         // Measured on MI355X at 768³: 4 rows (96 VGPRs) beats 6 (141) and 8 (178)
         // for the aligned kernel; the overlap kernel peaks at 6.
         return kernel == DP_KERNEL_FUSED_OVERLAP ? 6 : 4;
@@ -236,15 +242,15 @@ struct dp_sim {
               "hipMemcpyAsync");
     }
 
-    void fill_tiles(StepArgs &a, int r, uint32_t owned_x) const {
+    void fill_tiles(StepArgs &a, int r, uint32_t owned_x, const Tiling &t) const {
         const uint32_t nx = grid.extent[0], ny = grid.extent[1], nz = grid.extent[2];
         a.tiles_x = (nx + owned_x - 1) / owned_x;
         a.tiles_y = (ny + (r - 2) - 1) / (r - 2);
-        a.depth = std::min<uint32_t>(nz, depth ? depth : 32);
+        a.depth = std::min<uint32_t>(nz, t.depth ? t.depth : 32);
         a.tiles_z = (nz + a.depth - 1) / a.depth;
         a.tiles_total = a.tiles_x * a.tiles_y * a.tiles_z;
-        if (tile_flags & DP_TILE_XCD_REMAP) {
-            const uint32_t group = (uint32_t(tile_flags) >> 8) & 0xFFu;
+        if (t.flags & DP_TILE_XCD_REMAP) {
+            const uint32_t group = (uint32_t(t.flags) >> 8) & 0xFFu;
             a.xcd_remap = group ? group : 8;
         } else {
             a.xcd_remap = 0;
@@ -267,12 +273,13 @@ struct dp_sim {
                 a.n_magnetic = t.magnetic;
                 a.n_electric = t.electric;
                 a.intensity = accumulating ? intensity.as<double>() : nullptr;
+                const Tiling &t = active();
                 if (kernel == DP_KERNEL_FUSED) {
-                    fill_tiles(a, r, kAlignedX);
-                    launch_fused_aligned(a, r, (tile_flags & DP_TILE_NONTEMPORAL) != 0,
-                                         (tile_flags & DP_TILE_BUFFER) != 0, stream);
+                    fill_tiles(a, r, kAlignedX, t);
+                    launch_fused_aligned(a, r, (t.flags & DP_TILE_NONTEMPORAL) != 0, (t.flags & DP_TILE_BUFFER) != 0,
+                                         stream);
                 } else {
-                    fill_tiles(a, r, kOwnedX);
+                    fill_tiles(a, r, kOwnedX, t);
                     launch_fused(a, r, stream);
                 }
                 current = 1 - current;

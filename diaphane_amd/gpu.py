@@ -221,6 +221,13 @@ class Simulation:
         """Fused-kernel tiling; flags: 1 = XCD-contiguous tiles, 2 = non-temporal stores."""
         check(lib().dp_sim_set_tile(self._handle, rows, depth, flags))
 
+    def autotune(self, trials: int = 3) -> dict:
+        """Times the fused kernel's tile candidates on this grid and keeps the
+        fastest; the fields are unchanged."""
+        rows, depth, flags, ms = ctypes.c_int(), ctypes.c_int(), ctypes.c_int(), c_float()
+        check(lib().dp_sim_autotune(self._handle, trials, byref(rows), byref(depth), byref(flags), byref(ms)))
+        return {"rows": rows.value, "depth": depth.value, "flags": flags.value, "ms_per_step": ms.value}
+
     def record_event(self, slot: int) -> None:
         check(lib().dp_sim_record_event(self._handle, slot))
 

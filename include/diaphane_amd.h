@@ -289,6 +289,15 @@ int dp_sim_set_kernel(dp_sim *sim, int kernel);
  * 0 for rows/depth selects the tuned default; flags < 0 keeps the current
  * flags. Results never depend on any of it. */
 int dp_sim_set_tile(dp_sim *sim, int rows, int depth, int flags);
+/* Autotuner for DP_KERNEL_FUSED: times `trials` launches of every rows
+ * (4, 6, 8) × depth (16, 32, 64) × addressing (pointer, buffer where it
+ * fits) candidate on this sim's grid, keeps the fastest and reports it (any
+ * out pointer may be null). A sim keeps one tiling for plain steps and one
+ * for intensity-accumulating steps; the autotuner sets the one for the
+ * sim's current mode (dp_sim_set_tile sets both). The state is unchanged:
+ * trial launches read the current fields, write only the idle half of each
+ * ping-pong pair, and accumulate into a scratch buffer. Synchronous. */
+int dp_sim_autotune(dp_sim *sim, uint32_t trials, int *rows, int *depth, int *flags, float *ms_per_step);
 
 /* HIP-event timing on the sim's stream: record event `slot` (0..7) now;
  * elapsed milliseconds between two recorded slots (waits for the later). */

@@ -259,6 +259,26 @@ def test_fused_tiles_agree_bit_for_bit(context):
         assert np.array_equal(h, results[0][1])
 
 
+def test_autotuning_leaves_the_fields_unchanged(context):
+    scene = seam_scene()
+    tuned = Simulation(context, scene)
+    tuned.advance_by(50)
+    choice = tuned.autotune(2)
+    assert choice["rows"] in (4, 6, 8) and choice["depth"] in (16, 32, 64) and choice["ms_per_step"] > 0.0
+    tuned.advance_by(30)
+    tuned.set_accumulate_intensity(True)
+    tuned.advance_by(10)
+    tuned.autotune(2)  # the accumulating steps' shape, timed into a scratch buffer
+    tuned.advance_by(20)
+    plain = Simulation(context, scene)
+    plain.advance_by(80)
+    plain.set_accumulate_intensity(True)
+    plain.advance_by(30)
+    assert np.array_equal(tuned.read_electric(), plain.read_electric())
+    assert np.array_equal(tuned.read_magnetic(), plain.read_magnetic())
+    assert np.array_equal(tuned.read_intensity_f64(), plain.read_intensity_f64())
+
+
 def test_energy_matches_the_oracle(context):
     scene = slab_and_conductor_scene()
     reference, candidate = run_pair(context, scene, 120, KERNEL_FUSED)
