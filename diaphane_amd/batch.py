@@ -77,7 +77,7 @@ class Batch:
         rows, depth, flags = c_int(), c_int(), c_int()
         check(lib().dp_batch_tiling(self._handle, byref(rows), byref(depth), byref(flags)))
         return {"rows": rows.value, "depth": depth.value, "wave_per_tile": bool(flags.value & WAVE_PER_TILE),
-                "nontemporal": bool(flags.value & 2)}
+                "buffer": bool(flags.value & 4), "nontemporal": bool(flags.value & 2)}
 
     def __len__(self) -> int:
         return int(lib().dp_batch_size(self._handle))

@@ -94,6 +94,13 @@ struct dp_batch {
     }
 
     bool direct() const { return (flags & DP_BATCH_WAVE_PER_TILE) != 0; }
+    // Buffer addressing when asked for and every member's offsets fit it.
+    bool buffer() const {
+        if (!(flags & DP_TILE_BUFFER)) return false;
+        for (const dp_sim *s : members)
+            if (!buffer_addressing_fits(s->cells, s->grid.extent[0], s->grid.extent[1])) return false;
+        return true;
+    }
 
     size_t index_of(const dp_sim *sim) const {
         for (size_t i = 0; i < members.size(); ++i)
@@ -180,7 +187,7 @@ struct dp_batch {
         const bool nt = (flags & DP_TILE_NONTEMPORAL) != 0;
         auto launch = [&] {
             launch_fused_batch(instance_table.as<InstanceDesc>(), trial_slots.as<StepSlot>(), owners.as<uint32_t>(),
-                               n_tiles, trial_injections.as<InjDesc>(), rows, nt, direct(), blocks, stream);
+                               n_tiles, trial_injections.as<InjDesc>(), rows, nt, buffer(), direct(), blocks, stream);
         };
         launch();  // first launch of a shape loads its code object
         check(hipGetLastError(), "kernel launch");
@@ -198,15 +205,17 @@ struct dp_batch {
     void autotune(uint32_t trials, int *rows_out, int *depth_out, float *ms_out) {
         static const int row_choices[] = {4, 6, 8};
         static const uint32_t depth_choices[] = {8, 16, 32, 64};
-        static const int grid_choices[] = {0, DP_BATCH_WAVE_PER_TILE};
-        int best_rows = rows, best_grid = flags & DP_BATCH_WAVE_PER_TILE;
+        static const int grid_choices[] = {0, DP_BATCH_WAVE_PER_TILE, DP_TILE_BUFFER,
+                                           DP_BATCH_WAVE_PER_TILE | DP_TILE_BUFFER};
+        const int tuned_bits = DP_BATCH_WAVE_PER_TILE | DP_TILE_BUFFER;
+        int best_rows = rows, best_grid = flags & tuned_bits;
         uint32_t best_depth = depth;
         float best = std::numeric_limits<float>::infinity();
         if (!members.empty()) {
             for (int g : grid_choices)
                 for (int r : row_choices)
                     for (uint32_t d : depth_choices) {
-                        flags = (flags & ~DP_BATCH_WAVE_PER_TILE) | g;
+                        flags = (flags & ~tuned_bits) | g;
                         rows = r;
                         depth = d;
                         dirty = true;
@@ -219,7 +228,7 @@ struct dp_batch {
                         }
                     }
         }
-        flags = (flags & ~DP_BATCH_WAVE_PER_TILE) | best_grid;
+        flags = (flags & ~tuned_bits) | best_grid;
         rows = best_rows;
         depth = best_depth;
         dirty = true;
@@ -236,7 +245,7 @@ struct dp_batch {
         auto launch_all = [&] {
             for (uint32_t s = 0; s < steps; ++s)
                 launch_fused_batch(instance_table.as<InstanceDesc>(), slots + size_t(s) * members.size(),
-                                   owners.as<uint32_t>(), n_tiles, inj, rows, nt, direct(), blocks, stream);
+                                   owners.as<uint32_t>(), n_tiles, inj, rows, nt, buffer(), direct(), blocks, stream);
         };
         if (graph_steps == 0 || steps != graph_steps) {
             launch_all();
@@ -444,7 +453,7 @@ int dp_batch_set_tile(dp_batch *batch, int rows, int depth, int flags) {
         batch->sync();
         batch->rows = rows ? rows : 4;
         batch->depth = depth ? static_cast<uint32_t>(depth) : 16;
-        if (flags >= 0) batch->flags = flags & (DP_TILE_NONTEMPORAL | DP_BATCH_WAVE_PER_TILE);
+        if (flags >= 0) batch->flags = flags & (DP_TILE_NONTEMPORAL | DP_TILE_BUFFER | DP_BATCH_WAVE_PER_TILE);
         batch->dirty = true;
     });
 }

@@ -532,35 +532,49 @@ static void launch_fused_aligned_as(const StepArgs &args, int rows, bool nontemp
 #undef DPA_ROWS
 }
 
+template <int R, typename Access>
+static void launch_batch_as(const InstanceDesc *instances, const StepSlot *slots, const uint32_t *owners,
+                            uint32_t n_tiles, const InjDesc *injections, bool direct, uint32_t blocks,
+                            hipStream_t stream) {
+    if (direct)
+        k_fused_batch_direct<R, Access><<<(n_tiles + 3) / 4, 256, 0, stream>>>(instances, slots, owners, n_tiles,
+                                                                               injections);
+    else
+        k_fused_batch<R, Access><<<blocks, 256, 0, stream>>>(instances, slots, owners, n_tiles, injections);
+}
+
 template <int R>
 static void launch_batch_rows(const InstanceDesc *instances, const StepSlot *slots, const uint32_t *owners,
-                              uint32_t n_tiles, const InjDesc *injections, bool nontemporal, bool direct,
+                              uint32_t n_tiles, const InjDesc *injections, bool nontemporal, bool buffer, bool direct,
                               uint32_t blocks, hipStream_t stream) {
-    if (direct) {
-        const uint32_t grid = (n_tiles + 3) / 4;
+    if (buffer) {
         if (nontemporal)
-            k_fused_batch_direct<R, true><<<grid, 256, 0, stream>>>(instances, slots, owners, n_tiles, injections);
+            launch_batch_as<R, BufferAccess<true>>(instances, slots, owners, n_tiles, injections, direct, blocks, stream);
         else
-            k_fused_batch_direct<R, false><<<grid, 256, 0, stream>>>(instances, slots, owners, n_tiles, injections);
+            launch_batch_as<R, BufferAccess<false>>(instances, slots, owners, n_tiles, injections, direct, blocks,
+                                                    stream);
     } else if (nontemporal) {
-        k_fused_batch<R, true><<<blocks, 256, 0, stream>>>(instances, slots, owners, n_tiles, injections);
+        launch_batch_as<R, Pointer32<true>>(instances, slots, owners, n_tiles, injections, direct, blocks, stream);
     } else {
-        k_fused_batch<R, false><<<blocks, 256, 0, stream>>>(instances, slots, owners, n_tiles, injections);
+        launch_batch_as<R, Pointer32<false>>(instances, slots, owners, n_tiles, injections, direct, blocks, stream);
     }
 }
 
 void launch_fused_batch(const InstanceDesc *instances, const StepSlot *slots, const uint32_t *owners,
-                        uint32_t n_tiles, const InjDesc *injections, int rows, bool nontemporal, bool direct,
-                        uint32_t blocks, hipStream_t stream) {
+                        uint32_t n_tiles, const InjDesc *injections, int rows, bool nontemporal, bool buffer,
+                        bool direct, uint32_t blocks, hipStream_t stream) {
     switch (rows) {
     case 6:
-        launch_batch_rows<6>(instances, slots, owners, n_tiles, injections, nontemporal, direct, blocks, stream);
+        launch_batch_rows<6>(instances, slots, owners, n_tiles, injections, nontemporal, buffer, direct, blocks,
+                             stream);
         break;
     case 8:
-        launch_batch_rows<8>(instances, slots, owners, n_tiles, injections, nontemporal, direct, blocks, stream);
+        launch_batch_rows<8>(instances, slots, owners, n_tiles, injections, nontemporal, buffer, direct, blocks,
+                             stream);
         break;
     default:
-        launch_batch_rows<4>(instances, slots, owners, n_tiles, injections, nontemporal, direct, blocks, stream);
+        launch_batch_rows<4>(instances, slots, owners, n_tiles, injections, nontemporal, buffer, direct, blocks,
+                             stream);
         break;
     }
 }
@@ -572,9 +586,15 @@ uint32_t batch_resident_blocks(int rows, bool nontemporal) {
     if (hipGetDeviceProperties(&prop, device) != hipSuccess) return 1024;
     const void *kernel = nullptr;
     switch (rows) {
-    case 6: kernel = nontemporal ? (const void *)k_fused_batch<6, true> : (const void *)k_fused_batch<6, false>; break;
-    case 8: kernel = nontemporal ? (const void *)k_fused_batch<8, true> : (const void *)k_fused_batch<8, false>; break;
-    default: kernel = nontemporal ? (const void *)k_fused_batch<4, true> : (const void *)k_fused_batch<4, false>; break;
+    case 6:
+        kernel = nontemporal ? (const void *)k_fused_batch<6, Pointer32<true>> : (const void *)k_fused_batch<6, Pointer32<false>>;
+        break;
+    case 8:
+        kernel = nontemporal ? (const void *)k_fused_batch<8, Pointer32<true>> : (const void *)k_fused_batch<8, Pointer32<false>>;
+        break;
+    default:
+        kernel = nontemporal ? (const void *)k_fused_batch<4, Pointer32<true>> : (const void *)k_fused_batch<4, Pointer32<false>>;
+        break;
     }
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, 256, 0) != hipSuccess || per_cu <= 0)
         per_cu = 4;

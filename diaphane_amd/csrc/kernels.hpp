@@ -99,9 +99,11 @@ bool buffer_addressing_fits(uint64_t cells, uint32_t nx, uint32_t ny);
 // One launch advancing every active instance of a batch by one step.
 // `blocks` is the persistent grid size (workgroups of 4 waves); with
 // `direct` the launch instead has one wave per tile.
+// `buffer` selects buffer-resource addressing; the host guarantees that every
+// instance's offsets fit it.
 void launch_fused_batch(const InstanceDesc *instances, const StepSlot *slots, const uint32_t *owners,
-                        uint32_t n_tiles, const InjDesc *injections, int rows, bool nontemporal, bool direct,
-                        uint32_t blocks, hipStream_t stream);
+                        uint32_t n_tiles, const InjDesc *injections, int rows, bool nontemporal, bool buffer,
+                        bool direct, uint32_t blocks, hipStream_t stream);
 // Workgroups of the batch kernel that fit on the device at once.
 uint32_t batch_resident_blocks(int rows, bool nontemporal);
 void launch_update_h(const StepArgs &args, int direction, hipStream_t stream);

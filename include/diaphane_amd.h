@@ -345,15 +345,16 @@ int dp_batch_wait(dp_batch *batch, uint32_t timeout_ms);
  * directly). Results never depend on it. */
 int dp_batch_set_graph_steps(dp_batch *batch, uint32_t steps);
 /* Rows per wave (4, 6 or 8; 0 = default 4), z-slab depth (0 = 16) and
- * flags (< 0 keeps the current ones): DP_TILE_NONTEMPORAL, and
- * DP_BATCH_WAVE_PER_TILE, which launches one wave per tile instead of the
- * persistent grid whose waves stride over the tile list. The default is
- * both flags set. */
+ * flags (< 0 keeps the current ones): DP_TILE_NONTEMPORAL, DP_TILE_BUFFER
+ * (used when every member's offsets fit), and DP_BATCH_WAVE_PER_TILE, which
+ * launches one wave per tile instead of the persistent grid whose waves
+ * stride over the tile list. The default is DP_TILE_NONTEMPORAL |
+ * DP_BATCH_WAVE_PER_TILE. */
 enum { DP_BATCH_WAVE_PER_TILE = 16 };
 int dp_batch_set_tile(dp_batch *batch, int rows, int depth, int flags);
-/* Autotuner: times `trials` launches of every grid form × rows × depth
- * candidate (persistent or one wave per tile; rows 4, 6, 8; depth 8, 16,
- * 32, 64) on the members as they are, keeps the fastest and reports rows,
+/* Autotuner: times `trials` launches of every grid form × addressing ×
+ * rows × depth candidate (persistent or one wave per tile; pointer or
+ * buffer; rows 4, 6, 8; depth 8, 16, 32, 64) on the members as they are, keeps the fastest and reports rows,
  * depth and time (any out pointer may be null). Member state is unchanged:
  * trial launches write only the idle half of each ping-pong pair.
  * Synchronous. */

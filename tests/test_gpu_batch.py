@@ -175,9 +175,10 @@ def test_a_member_refuses_solo_advance_and_leaves_when_destroyed(context):
     batch.wait()
 
 
-@pytest.mark.parametrize("rows,depth,flags", [(6, 16, 2), (8, 5, 0), (4, 64, 2), (4, 32, 18), (6, 7, 16)])
+@pytest.mark.parametrize("rows,depth,flags", [(6, 16, 2), (8, 5, 0), (4, 64, 2), (4, 32, 18), (6, 7, 16), (4, 16, 22),
+                                             (6, 9, 6), (8, 32, 20)])
 def test_tile_shapes_do_not_change_results(context, rows, depth, flags):
-    # flags 16: one wave per tile instead of the persistent grid
+    # flags 16: one wave per tile instead of the persistent grid; 4: buffer addressing
     instances = batched_uneven(3, seed=14, lo=22, hi=80)
     batch = Batch(context)
     batch.set_tile(rows, depth, flags)
