@@ -82,10 +82,13 @@ struct dp_batch {
         sim->batch = nullptr;
     }
 
-    void detach(dp_sim *sim) {
+    // Removes a member. Never throws: it runs from dp_sim's destructor. A
+    // failed wait here leaves the error to the stream's next checked call;
+    // dp_batch_remove waits (checked) before it gets here.
+    void detach(dp_sim *sim) noexcept {
         const size_t i = index_of(sim);
         if (i == members.size()) return;
-        sync();
+        (void)hipStreamSynchronize(stream);
         for (auto &set : sets) set.in_flight = false;
         members.erase(members.begin() + static_cast<std::ptrdiff_t>(i));
         targets.erase(targets.begin() + static_cast<std::ptrdiff_t>(i));
@@ -156,12 +159,9 @@ struct dp_batch {
         instance_table.allocate(std::max<size_t>(table.size(), 1) * sizeof(InstanceDesc));
         owners.allocate(std::max<size_t>(owner.size(), 1) * sizeof(uint32_t));
         if (!table.empty())
-            check(hipMemcpy(instance_table.ptr, table.data(), table.size() * sizeof(InstanceDesc),
-                            hipMemcpyHostToDevice),
-                  "hipMemcpy");
+            copy_on(stream, instance_table.ptr, table.data(), table.size() * sizeof(InstanceDesc), hipMemcpyHostToDevice);
         if (!owner.empty())
-            check(hipMemcpy(owners.ptr, owner.data(), owner.size() * sizeof(uint32_t), hipMemcpyHostToDevice),
-                  "hipMemcpy");
+            copy_on(stream, owners.ptr, owner.data(), owner.size() * sizeof(uint32_t), hipMemcpyHostToDevice);
         const uint32_t resident = batch_resident_blocks(rows, (flags & DP_TILE_NONTEMPORAL) != 0);
         blocks = std::max<uint32_t>(1, std::min<uint32_t>(resident, (n_tiles + 3) / 4));
         ++version;
@@ -181,8 +181,7 @@ struct dp_batch {
             slot[i].parity = static_cast<uint32_t>(members[i]->current);
         }
         trial_slots.allocate(std::max<size_t>(slot.size(), 1) * sizeof(StepSlot));
-        check(hipMemcpy(trial_slots.ptr, slot.data(), slot.size() * sizeof(StepSlot), hipMemcpyHostToDevice),
-              "hipMemcpy");
+        copy_on(stream, trial_slots.ptr, slot.data(), slot.size() * sizeof(StepSlot), hipMemcpyHostToDevice);
         if (!trial_injections.ptr) trial_injections.allocate(sizeof(InjDesc));
         const bool nt = (flags & DP_TILE_NONTEMPORAL) != 0;
         auto launch = [&] {
@@ -255,7 +254,9 @@ struct dp_batch {
         if (!set.graph || set.graph_version != version || set.graph_steps != steps) {
             set.drop_graph();
             hipGraph_t graph = nullptr;
-            check(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+            // Relaxed: calls other threads make meanwhile (a server's builders
+            // allocating, a client copying a result out) cannot invalidate it.
+            check(hipStreamBeginCapture(stream, hipStreamCaptureModeRelaxed), "hipStreamBeginCapture");
             launch_all();
             check(hipStreamEndCapture(stream, &graph), "hipStreamEndCapture");
             const hipError_t made = hipGraphInstantiate(&set.graph, graph, nullptr, nullptr, 0);
@@ -391,6 +392,7 @@ int dp_batch_remove(dp_batch *batch, dp_sim *sim) {
         if (!batch || !sim) fail(DP_ERR_INVALID, "null argument");
         const size_t i = batch->index_of(sim);
         if (i == batch->members.size()) fail(DP_ERR_INVALID, "the sim is not a member of this batch");
+        batch->sync();  // a pending device error surfaces here, as a status
         batch->detach(sim);
     });
 }

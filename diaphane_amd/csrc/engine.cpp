@@ -316,8 +316,7 @@ static int read_field(dp_sim *sim, const DeviceBuffer *pair, float *out, size_t 
         if (!sim || !out) fail(DP_ERR_INVALID, "null argument");
         if (count != 3 * sim->cells) fail(DP_ERR_SIZE, "buffer length must be 3 x cell count");
         sim->sync();
-        check(hipMemcpy(out, pair[sim->current].ptr, count * sizeof(float), hipMemcpyDeviceToHost),
-              "hipMemcpy");
+        copy_on(sim->stream, out, pair[sim->current].ptr, count * sizeof(float), hipMemcpyDeviceToHost);
     });
 }
 
@@ -338,8 +337,7 @@ int dp_sim_read_intensity_f64(dp_sim *sim, double *out, size_t count) {
             std::fill(out, out + count, 0.0);
             return;
         }
-        check(hipMemcpy(out, sim->intensity.ptr, count * sizeof(double), hipMemcpyDeviceToHost),
-              "hipMemcpy");
+        copy_on(sim->stream, out, sim->intensity.ptr, count * sizeof(double), hipMemcpyDeviceToHost);
     });
 }
 
@@ -360,10 +358,8 @@ int dp_sim_write_fields(dp_sim *sim, const float *electric, const float *magneti
         sim->clear_intensity();  // the average belongs to the abandoned history
         if (count != 3 * sim->cells) fail(DP_ERR_SIZE, "snapshot was taken from a different grid");
         sim->sync();
-        check(hipMemcpy(sim->e[sim->current].ptr, electric, count * sizeof(float), hipMemcpyHostToDevice),
-              "hipMemcpy");
-        check(hipMemcpy(sim->h[sim->current].ptr, magnetic, count * sizeof(float), hipMemcpyHostToDevice),
-              "hipMemcpy");
+        copy_on(sim->stream, sim->e[sim->current].ptr, electric, count * sizeof(float), hipMemcpyHostToDevice);
+        copy_on(sim->stream, sim->h[sim->current].ptr, magnetic, count * sizeof(float), hipMemcpyHostToDevice);
         sim->step = step;
     });
 }
@@ -380,9 +376,7 @@ int dp_sim_energy(dp_sim *sim, double *electric, double *magnetic) {
         check(hipGetLastError(), "kernel launch");
         std::vector<double> partials(2 * blocks);
         sim->sync();
-        check(hipMemcpy(partials.data(), sim->partials.ptr, partials.size() * sizeof(double),
-                        hipMemcpyDeviceToHost),
-              "hipMemcpy");
+        copy_on(sim->stream, partials.data(), sim->partials.ptr, partials.size() * sizeof(double), hipMemcpyDeviceToHost);
         double ue = 0.0, uh = 0.0;
         for (uint32_t b = 0; b < blocks; ++b) {
             ue += partials[2 * b];

@@ -50,6 +50,14 @@ inline int guarded(F &&body) {
 
 [[noreturn]] inline void fail(int code, const std::string &message) { throw Error{code, message}; }
 
+// A copy ordered on `stream`, waited for. The library never uses the legacy
+// NULL stream: HIP refuses a legacy-stream copy while any stream of the
+// device is capturing a graph (a batch on another thread may be).
+inline void copy_on(hipStream_t stream, void *dst, const void *src, size_t bytes, hipMemcpyKind kind) {
+    check(hipMemcpyAsync(dst, src, bytes, kind, stream), "hipMemcpyAsync");
+    check(hipStreamSynchronize(stream), "hipStreamSynchronize");
+}
+
 // Device allocation owned by a sim.
 struct DeviceBuffer {
     void *ptr = nullptr;

@@ -234,7 +234,7 @@ def batched_uneven(count: int = 256, seed: int = 4, lo: int = 48, hi: int = 160,
             scene.shapes.append(Shape.Block(centre, dims, m))
         modes = []
         for _ in range(4):
-            mode = (rng.integer(1, 3), rng.integer(1, 3), rng.integer(1, 3))
+            mode = (rng.integer(1, 2), rng.integer(1, 2), rng.integer(1, 2))
             amplitude = (rng.uniform(-1, 1), rng.uniform(-1, 1), rng.uniform(-1, 1))
             modes.append((mode, amplitude, rng.uniform(0.0, 2.0 * math.pi)))
         out.append(Instance(scene, modes))

@@ -66,6 +66,29 @@ def test_each_result_is_the_same_alone_and_in_the_mix(context, one_at_a_time):
         assert server.stats()["used_bytes"] == 0
 
 
+def test_results_read_while_other_requests_run(context):
+    # A client copying a finished result to the host while the loop keeps
+    # capturing and launching graphs for the others: the library's copies
+    # are stream-ordered, never on the legacy stream (which HIP refuses
+    # while a stream is capturing).
+    stream = served_stream(8, rate=200.0, seed=25, lo=32, hi=48, steps_lo=40, steps_hi=400)
+    alone = [solo_fields(context, r) for r in stream]
+    with Server(context, chunk_steps=8) as server:
+        tickets = {server.submit(r.scene, r.steps): k for k, r in enumerate(stream)}
+        read_while_running = 0
+        while tickets:
+            for ticket in list(tickets):
+                if server.info(ticket).state == DONE:
+                    k = tickets.pop(ticket)
+                    result = server.result(ticket)
+                    read_while_running += server.stats()["running"] > 0
+                    assert bits_equal(result.read_electric(), alone[k][0]), f"request {k}: E differs"
+                    assert bits_equal(result.read_magnetic(), alone[k][1]), f"request {k}: H differs"
+                    server.release(ticket)
+            time.sleep(0.001)
+        assert read_while_running > 0
+
+
 def test_served_results_match_the_oracle(context):
     stream = small_stream(3, seed=22)
     with Server(context, chunk_steps=8) as server:
