@@ -365,8 +365,8 @@ def main() -> int:
     args = parser.parse_args()
 
     if args.gpus != 1:
-        print(json.dumps({"error": "this engine runs on one GPU; multi-GPU slab decomposition is "
-                                   "design-only (see DESIGN.md)", "n_gpus": args.gpus}))
+        print(json.dumps({"error": "this engine runs on one GPU; splitting a grid into z-slabs across "
+                                   "GPUs is not built", "n_gpus": args.gpus}))
         return 2
 
     if args.workload == "batched_uneven_256":
