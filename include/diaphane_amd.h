@@ -409,7 +409,8 @@ void dp_server_destroy(dp_server *server);
 /* Copies the scene; DP_ERR_BUDGET if the request alone exceeds the budget. */
 int dp_server_submit(dp_server *server, const dp_scene *scene, uint64_t steps, uint64_t *ticket);
 int dp_server_query(dp_server *server, uint64_t ticket, dp_request_info *info);
-/* Blocks until the request is done; its failure status if it failed. */
+/* Blocks until the request is done; its failure status if it failed. A
+ * ticket must not be released while another thread waits on it. */
 int dp_server_wait(dp_server *server, uint64_t ticket, uint32_t timeout_ms);
 /* The finished request's sim, borrowed until dp_server_release. */
 int dp_server_result(dp_server *server, uint64_t ticket, dp_sim **sim);
