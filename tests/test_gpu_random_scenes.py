@@ -1,0 +1,99 @@
+"""Seeded random scenes on the device against the oracle, bit for bit.
+
+Each draw (grid, boundary, materials, shapes, sources: see random_scenes.py)
+runs through a kernel path and a tiling chosen from the same seed, with the
+|E|² accumulator switched on halfway for every other draw; the fields and the
+f64 intensity must equal the oracle's exactly. A second test advances
+sixteen draws together in one batch, each to its own step count.
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+import oracle
+from diaphane_amd.batch import Batch
+from diaphane_amd.gpu import KERNEL_FUSED, KERNEL_FUSED_OVERLAP, KERNEL_TWO_PASS, TILE_BUFFER, TILE_NONTEMPORAL, \
+    TILE_XCD_REMAP, Simulation
+
+from random_scenes import random_case
+
+pytestmark = pytest.mark.gpu
+
+SOLO_SEEDS = range(100, 124)
+BATCH_SEEDS = range(200, 216)
+
+
+def bits_equal(a: np.ndarray, b: np.ndarray) -> bool:
+    return np.array_equal(a.view(np.uint32), b.view(np.uint32))
+
+
+def run_plan(seed: int):
+    """(kernel, rows, depth, flags, accumulate) drawn from the seed."""
+    rng = np.random.default_rng(10_000 + seed)
+    kernel = (KERNEL_FUSED, KERNEL_FUSED, KERNEL_TWO_PASS, KERNEL_FUSED_OVERLAP)[int(rng.integers(4))]
+    rows = int(rng.choice([4, 6, 8, 10, 12]))
+    depth = int(rng.choice([0, 5, 16, 33, 64]))
+    flags = int(rng.choice([0, TILE_NONTEMPORAL, TILE_BUFFER, TILE_NONTEMPORAL | TILE_BUFFER]))
+    if rng.random() < 0.3:
+        flags |= TILE_XCD_REMAP | (int(rng.choice([0, 2, 4])) << 8)
+    return kernel, rows, depth, flags, bool(seed % 2)
+
+
+def assert_matches_oracle(case, reference, device, intensity: bool):
+    label = case.describe()
+    assert reference.step_count() == device.step_count() == case.steps
+    electric, magnetic = device.read_electric(), device.read_magnetic()
+    assert reference.is_finite(), f"{label}: the oracle diverged"
+    assert np.abs(reference.electric_all).max() > 0.0, f"{label}: nothing was injected"
+    for name, ref_all, cand in (("E", reference.electric_all, electric), ("H", reference.magnetic_all, magnetic)):
+        diff = np.flatnonzero(ref_all.view(np.uint32) != cand.view(np.uint32))
+        assert diff.size == 0, (f"{label} {name}: {diff.size} cells differ; first at {diff[0]}: "
+                                f"oracle {ref_all[diff[0]]!r} vs device {cand[diff[0]]!r}")
+    if intensity:
+        assert reference.accumulated_steps() == device.accumulated_steps()
+        assert np.array_equal(reference.intensity_f64(), device.read_intensity_f64()), f"{label}: intensity"
+
+
+@pytest.mark.parametrize("seed", SOLO_SEEDS)
+def test_a_random_scene_matches_the_oracle(context, seed):
+    case = random_case(seed)
+    kernel, rows, depth, flags, intensity = run_plan(seed)
+    print(f"{case.describe()} kernel {kernel} rows {rows} depth {depth} flags {flags:#x}", flush=True)
+    reference = oracle.Simulation(case.scene)
+    device = Simulation(context, case.scene)
+    device.set_kernel(kernel)
+    device.set_tile(rows, depth, flags)
+    half = case.steps // 2
+    for sim in (reference, device):
+        sim.advance_by(half)
+        if intensity:
+            sim.set_accumulate_intensity(True)
+        sim.advance_by(case.steps - half)
+    assert_matches_oracle(case, reference, device, intensity)
+    device.close()
+
+
+def test_random_scenes_batched_to_their_own_step_counts(context):
+    cases = [random_case(seed) for seed in BATCH_SEEDS]
+    batch = Batch(context)
+    batch.set_graph_steps(16)
+    sims = []
+    for k, case in enumerate(cases):
+        print(case.describe(), flush=True)
+        sim = Simulation(context, case.scene)
+        if k % 2:
+            sim.set_accumulate_intensity(True)
+        batch.add(sim, case.steps)
+        sims.append(sim)
+    batch.advance_by(max(case.steps for case in cases))
+    batch.wait()
+    for k, (case, sim) in enumerate(zip(cases, sims)):
+        reference = oracle.Simulation(case.scene)
+        if k % 2:
+            reference.set_accumulate_intensity(True)
+        reference.advance_by(case.steps)
+        assert_matches_oracle(case, reference, sim, bool(k % 2))
+    batch.close()
+    for sim in sims:
+        sim.close()
