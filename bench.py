@@ -402,6 +402,9 @@ def main() -> int:
             sim.set_accumulate_intensity(True)  # tune the accumulating steps' shape too
             tiling["with_intensity"] = sim.autotune(3)
             sim.set_accumulate_intensity(False)
+    # The plain steps' tiling run as a pure copy (same tiles, rows, halos and
+    # stores, no stencil): the ceiling for that tiling's memory traffic.
+    tile_gbs = sim.tile_probe(tiling["rows"] if tiling["rows"] in (4, 6, 8) else 0, tiling["depth"], 5)
 
     if args.warmup:
         sim.advance_by(args.warmup)
@@ -473,6 +476,8 @@ def main() -> int:
             "kernel": "k_fused_aligned",
             "stream_probe_gbs": probe_gbs,
             "frac_of_stream_probe": achieved / probe_gbs,
+            "tile_probe_gbs": tile_gbs,
+            "plain_frac_of_tile_probe": (bytes_a / (ms_a / 1e3) / 1e9) / tile_gbs if ms_a else None,
             "bytes_per_cell": {"plain": BYTES_PER_CELL, "with_intensity_f64": BYTES_PER_CELL_INTENSITY},
             "avg_launch_ms": {"plain": ms_a / max(phase_a, 1), "with_intensity": ms_b / max(phase_b, 1)},
             "phase_gbs": {

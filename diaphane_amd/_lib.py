@@ -96,6 +96,7 @@ def _declare(lib: ctypes.CDLL) -> None:
         "dp_sim_elapsed_ms": (c_int, [sim, c_int, c_int, POINTER(c_float)]),
         "dp_sim_launch_count": (c_uint64, [sim]),
         "dp_sim_stream_probe": (c_int, [sim, c_int, POINTER(c_double)]),
+        "dp_sim_tile_probe": (c_int, [sim, c_int, c_int, c_int, POINTER(c_double)]),
         "dp_batch_create": (c_int, [c_void_p, POINTER(c_void_p)]),
         "dp_batch_destroy": (None, [c_void_p]),
         "dp_batch_add": (c_int, [c_void_p, sim]),

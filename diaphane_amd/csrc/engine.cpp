@@ -496,6 +496,7 @@ int dp_sim_autotune(dp_sim *sim, uint32_t trials, int *rows, int *depth, int *fl
 int dp_sim_stream_probe(dp_sim *sim, int iterations, double *gbs) {
     return guarded([&] {
         if (!sim || !gbs || iterations <= 0) fail(DP_ERR_INVALID, "bad argument");
+        if (sim->batch) fail(DP_ERR_INVALID, "this sim is advanced by its batch");
         const int from = sim->current, to = 1 - sim->current;
         auto launch = [&] {
             launch_stream_probe(sim->e[from].as<float>(), sim->h[from].as<float>(), sim->material.as<uint8_t>(),
@@ -511,6 +512,31 @@ int dp_sim_stream_probe(dp_sim *sim, int iterations, double *gbs) {
         check(hipEventElapsedTime(&ms, sim->events[6], sim->events[7]), "hipEventElapsedTime");
         const double bytes = 49.0 * double(sim->cells / 4 * 4) * iterations;
         *gbs = bytes / (double(ms) / 1e3) / 1e9;
+    });
+}
+
+int dp_sim_tile_probe(dp_sim *sim, int rows, int depth, int iterations, double *gbs) {
+    return guarded([&] {
+        if (!sim || !gbs || iterations <= 0 || depth < 0) fail(DP_ERR_INVALID, "bad argument");
+        if (sim->batch) fail(DP_ERR_INVALID, "this sim is advanced by its batch");
+        if (rows != 0 && rows != 4 && rows != 6 && rows != 8) fail(DP_ERR_INVALID, "rows must be 4, 6 or 8 (or 0)");
+        const dp_sim::Tiling &plain = sim->tiling[0];
+        dp_sim::Tiling t;
+        t.rows = rows ? rows : (plain.rows == 6 || plain.rows == 8 ? plain.rows : 4);
+        t.depth = depth ? uint32_t(depth) : plain.depth;
+        t.flags = 0;
+        StepArgs a = sim->args(sim->current, 1 - sim->current);
+        sim->fill_tiles(a, t.rows, kAlignedX, t);
+        auto launch = [&] { launch_tile_probe(a, t.rows, sim->stream); };
+        launch();
+        check(hipGetLastError(), "kernel launch");
+        check(hipEventRecord(sim->events[6], sim->stream), "hipEventRecord");
+        for (int i = 0; i < iterations; ++i) launch();
+        check(hipEventRecord(sim->events[7], sim->stream), "hipEventRecord");
+        check(hipEventSynchronize(sim->events[7]), "hipEventSynchronize");
+        float ms = 0.0f;
+        check(hipEventElapsedTime(&ms, sim->events[6], sim->events[7]), "hipEventElapsedTime");
+        *gbs = 49.0 * double(sim->cells) * iterations / (double(ms) / 1e3) / 1e9;
     });
 }
 

@@ -456,21 +456,31 @@ __device__ __forceinline__ void store_stream(float4 v, float4 *p) {
 }
 
 // One float4 of each of the six field streams (and 4 material bytes) per
-// thread, all loads issued before any store, stores non-temporal: the
-// fastest of the streaming forms measured on MI355X for this byte mix.
+// thread, stores non-temporal. Each load's address depends on the previous
+// load's value through `zero` (always 0), so a wave has one load in flight at
+// a time; launched at four workgroups per CU. Measured on MI355X this is the
+// fastest streaming form of the mix (about 6.2 TB/s): issuing all seven
+// loads at once, or running more waves per CU, congests the memory system
+// and streams about 12% slower.
 __global__ __launch_bounds__(256) void k_stream_probe(const float4 *__restrict__ e, const float4 *__restrict__ h,
                                                       const uint32_t *__restrict__ material,
                                                       float4 *__restrict__ e_out, float4 *__restrict__ h_out,
-                                                      uint64_t quads) {
+                                                      uint64_t quads, uint32_t zero) {
     const uint64_t q = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (q >= quads) return;
-    const float bias = float(material[q] & 1u);
     float4 a[3], b[3];
+    uint64_t dep = 0;
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
-        a[c] = e[c * quads + q];
-        b[c] = h[c * quads + q];
+        a[c] = e[c * quads + q + dep];
+        dep = __float_as_uint(a[c].x) & zero;
     }
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        b[c] = h[c * quads + q + dep];
+        dep = __float_as_uint(b[c].x) & zero;
+    }
+    const float bias = float(material[q + dep] & 1u);
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
         a[c].x += bias;
@@ -480,16 +490,88 @@ __global__ __launch_bounds__(256) void k_stream_probe(const float4 *__restrict__
     }
 }
 
+// The fused kernel's access pattern with no stencil: the same 64-aligned
+// tiles of R rows marching a z-slab, loading E at plane k+1, H and the
+// material at plane k for every row and the x-halo cells (lanes 0 and 63),
+// and storing E and H of the owned rows non-temporally. Its bandwidth is the
+// ceiling for that tiling's memory traffic; the stencil kernel can approach
+// it, not pass it.
+template <int R>
+__global__ __launch_bounds__(256) void k_tile_probe(StepArgs p) {
+    const uint32_t tile = __builtin_amdgcn_readfirstlane(blockIdx.x * 4 + threadIdx.x / kLanes);
+    if (tile >= p.tiles_total) return;
+    const int lane = threadIdx.x & (kLanes - 1);
+    const int nx = int(p.nx), ny = int(p.ny), nz = int(p.nz);
+    const uint64_t plane = uint64_t(p.nx) * p.ny, cells = p.cells;
+    const uint32_t tx = tile % p.tiles_x, rest = tile / p.tiles_x;
+    const uint32_t ty = rest % p.tiles_y, tz = rest / p.tiles_y;
+    const int x0 = int(tx) * kLanes, x = x0 + lane;
+    const int xc = x < nx ? x : nx - 1;
+    const bool is_lo = lane == 0, is_hi = lane == kLanes - 1;
+    const int xh = is_lo ? (x0 > 0 ? x0 - 1 : 0) : (is_hi ? (x0 + kLanes < nx ? x0 + kLanes : xc) : xc);
+    const int y0 = int(ty) * (R - 2) - 1;
+    const int zb = int(tz * p.depth);
+    const int ze = min(zb + int(p.depth), nz);
+    const int zs = zb > 0 ? zb - 1 : 0;
+    uint64_t own[R], halo[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const int y = y0 + r;
+        const int yc = y < 0 ? 0 : (y < ny ? y : ny - 1);
+        own[r] = uint64_t(yc) * uint64_t(nx) + uint64_t(xc);
+        halo[r] = uint64_t(yc) * uint64_t(nx) + uint64_t(xh);
+    }
+    for (int k = zs; k < ze; ++k) {
+        const uint64_t pk = uint64_t(k) * plane;
+        const uint64_t pn = uint64_t(k + 1 < nz ? k + 1 : k) * plane;
+        float ev[3][R], hv[3][R], eg[3][R], hg[3][R];
+        uint32_t mv[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                ev[c][r] = p.e_in[c * cells + pn + own[r]];
+                hv[c][r] = p.h_in[c * cells + pk + own[r]];
+                eg[c][r] = p.e_in[c * cells + pn + halo[r]];
+                hg[c][r] = p.h_in[c * cells + pk + halo[r]];
+            }
+            mv[r] = p.material[pk + own[r]];
+        }
+        if (k < zb) continue;
+#pragma unroll
+        for (int r = 1; r < R - 1; ++r) {
+            if (y0 + r >= ny || x >= nx) continue;
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                const float ve = ev[c][r] + hv[c][r - 1] + float(mv[r] & 1u) + (is_hi ? eg[c][r] : 0.0f);
+                const float vh = hv[c][r] + ev[c][r + 1] + (is_lo ? hg[c][r] : 0.0f);
+                __builtin_nontemporal_store(ve, p.e_out + c * cells + pk + own[r]);
+                __builtin_nontemporal_store(vh, p.h_out + c * cells + pk + own[r]);
+            }
+        }
+    }
+}
+
 }  // namespace
 
 void launch_stream_probe(const float *e, const float *h, const uint8_t *material, float *e_out, float *h_out,
                          uint64_t cells, hipStream_t stream) {
     const uint64_t quads = cells / 4;
     const uint32_t blocks = uint32_t((quads + 255) / 256);
-    k_stream_probe<<<blocks, 256, 0, stream>>>(reinterpret_cast<const float4 *>(e), reinterpret_cast<const float4 *>(h),
-                                               reinterpret_cast<const uint32_t *>(material),
-                                               reinterpret_cast<float4 *>(e_out), reinterpret_cast<float4 *>(h_out),
-                                               quads);
+    // 40 KiB of (unused) dynamic LDS per workgroup: four workgroups per CU.
+    k_stream_probe<<<blocks, 256, 40 * 1024, stream>>>(
+        reinterpret_cast<const float4 *>(e), reinterpret_cast<const float4 *>(h),
+        reinterpret_cast<const uint32_t *>(material), reinterpret_cast<float4 *>(e_out),
+        reinterpret_cast<float4 *>(h_out), quads, 0u);
+}
+
+void launch_tile_probe(const StepArgs &args, int rows, hipStream_t stream) {
+    const uint32_t blocks = (args.tiles_total + 3) / 4;
+    switch (rows) {
+    case 6: k_tile_probe<6><<<blocks, 256, 0, stream>>>(args); break;
+    case 8: k_tile_probe<8><<<blocks, 256, 0, stream>>>(args); break;
+    default: k_tile_probe<4><<<blocks, 256, 0, stream>>>(args); break;
+    }
 }
 
 void launch_fused(const StepArgs &args, int rows, hipStream_t stream) {

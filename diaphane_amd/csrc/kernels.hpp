@@ -120,5 +120,8 @@ uint32_t energy_blocks(uint32_t nx, uint32_t ny, uint32_t nz);
 // stencil: the practical bandwidth ceiling for the step's byte count.
 void launch_stream_probe(const float *e, const float *h, const uint8_t *material, float *e_out, float *h_out,
                          uint64_t cells, hipStream_t stream);
+// The fused kernel's tile access pattern (rows 4, 6 or 8; the tiling in
+// `args`) with no stencil: the ceiling for that tiling's memory traffic.
+void launch_tile_probe(const StepArgs &args, int rows, hipStream_t stream);
 
 }  // namespace dpa

@@ -242,5 +242,12 @@ class Simulation:
         check(lib().dp_sim_stream_probe(self._handle, iterations, byref(gbs)))
         return gbs.value
 
+    def tile_probe(self, rows: int = 0, depth: int = 0, iterations: int = 10) -> float:
+        """GB/s (49 B/cell) of the fused kernel's tile access pattern with no
+        stencil, at `rows` × `depth` (0: the plain tiling's)."""
+        gbs = c_double()
+        check(lib().dp_sim_tile_probe(self._handle, rows, depth, iterations, byref(gbs)))
+        return gbs.value
+
     def launch_count(self) -> int:
         return int(lib().dp_sim_launch_count(self._handle))

@@ -307,8 +307,18 @@ int dp_sim_elapsed_ms(dp_sim *sim, int from_slot, int to_slot, float *ms);
 /* Bandwidth (GB/s) of the step's access mix — E, H and the u8 material
  * read, E and H written, 49 bytes per cell — streamed with no stencil on
  * this sim's own buffers, averaged over `iterations` launches. The state is
- * left unchanged. The practical ceiling for the step's byte count. */
+ * left unchanged (only the idle half of each ping-pong pair is written).
+ * The practical ceiling for the step's byte count. No counterpart in the
+ * reference: a measurement aid. */
 int dp_sim_stream_probe(dp_sim *sim, int iterations, double *gbs);
+
+/* Bandwidth (GB/s, 49 bytes per cell) of the fused kernel's own access
+ * pattern with no stencil: the same 64-aligned tiles of `rows` rows (4, 6 or
+ * 8; 0 takes the sim's plain tiling) marching z-slabs of `depth` planes (0:
+ * the sim's), the same row and x-halo loads and owned-row stores, on this
+ * sim's buffers. The state is left unchanged. The ceiling for that tiling's
+ * memory traffic. No counterpart in the reference: a measurement aid. */
+int dp_sim_tile_probe(dp_sim *sim, int rows, int depth, int iterations, double *gbs);
 
 /* Forward-step launches enqueued so far (fused or H-kernel launches). */
 uint64_t dp_sim_launch_count(const dp_sim *sim);
