@@ -236,13 +236,14 @@ def bench_batched(args) -> int:
     return 0
 
 
-def serve_stream(context, stream, one_at_a_time: bool, chunk_steps: int, rows: int, depth: int) -> dict:
+def serve_stream(context, stream, one_at_a_time: bool, chunk_steps: int, rows: int, depth: int,
+                 chunk_cells: int = 0) -> dict:
     """Feeds the stream to a server at its arrival times; returns per-request
     latencies and the aggregate rate."""
     from diaphane_amd.serve import DONE, Server
 
     server = Server(context, one_at_a_time=one_at_a_time, chunk_steps=chunk_steps, rows=rows, depth=depth,
-                    host_threads=4)
+                    host_threads=4, chunk_cells=chunk_cells)
     pending = {}
     submitted_at = {}
     infos = []
@@ -305,10 +306,11 @@ def bench_served(args) -> int:
     log(f"{len(stream)} requests over {stream[-1].arrival_s:.1f} s, {updates / 1e9:.0f} G cell-updates")
     # warm the code paths (kernels load on first launch)
     warm = workloads.served_stream(3, 1000.0, seed=99, lo=32, hi=40, steps_lo=40, steps_hi=40)
-    serve_stream(context, warm, False, args.chunk_steps, args.rows, 0)
+    chunk_cells = (1 << 64) - 1 if args.chunk_cells < 0 else args.chunk_cells
+    serve_stream(context, warm, False, args.chunk_steps, args.rows, 0, chunk_cells)
     serve_stream(context, warm, True, args.chunk_steps, args.rows, 0)
     log("continuous batching")
-    batched = serve_stream(context, stream, False, args.chunk_steps, args.rows, 0)
+    batched = serve_stream(context, stream, False, args.chunk_steps, args.rows, 0, chunk_cells)
     log(f"  {batched['value'] / 1e9:.1f} G/s, p50 {batched['latency_s']['p50']:.2f} s")
     log("one at a time")
     sequential = serve_stream(context, stream, True, args.chunk_steps, args.rows, 0)
@@ -335,6 +337,7 @@ def bench_served(args) -> int:
             "kinds": kinds,
             "hbm_budget_bytes": 200 << 30,
             "chunk_steps": args.chunk_steps,
+            "chunk_cells": "every member" if args.chunk_cells < 0 else (args.chunk_cells or 32 << 20),
         },
         "served": batched,
         "sequential": sequential,
@@ -360,6 +363,9 @@ def main() -> int:
     parser.add_argument("--requests", type=int, default=200, help="served_stream: number of requests")
     parser.add_argument("--rate", type=float, default=50.0, help="served_stream: arrivals per second")
     parser.add_argument("--chunk-steps", type=int, default=32, help="served_stream: launches between joins")
+    parser.add_argument("--chunk-cells", type=int, default=0,
+                        help="served_stream: cells stepping per chunk, least remaining work first "
+                             "(0: the server's default; -1: every member every chunk)")
     parser.add_argument("--intensity-from", type=int, default=None,
                         help="step at which |E|² accumulation starts (-1: never; default: the workload's)")
     args = parser.parse_args()

@@ -15,7 +15,7 @@ from .scene import DpGridSpec, DpInjection, DpScene
 class DpServerConfig(ctypes.Structure):
     _fields_ = [("hbm_budget_bytes", c_uint64), ("max_resident", c_uint32), ("chunk_steps", c_uint32),
                 ("host_threads", c_uint32), ("one_at_a_time", ctypes.c_int32), ("rows", ctypes.c_int32),
-                ("depth", ctypes.c_int32)]
+                ("depth", ctypes.c_int32), ("chunk_cells", c_uint64)]
 
 
 class DpRequestInfo(ctypes.Structure):

@@ -47,9 +47,12 @@ struct dp_batch {
 
     // Defaults measured on MI355X over 256 uneven grids (48–160 cells a
     // side): one wave per tile beats the persistent grid by 13%, and 4 rows
-    // with 16-plane slabs is the fastest shape in that form.
+    // with 16-plane slabs is the fastest shape in that form. depth 0 sizes
+    // the slabs to the membership (auto_depth): a batch of one small grid
+    // needs short slabs to put enough waves on the device.
     int rows = 4;
-    uint32_t depth = 16;
+    uint32_t depth = 0;
+    uint32_t slab = 16;  // the slab depth the current tables use
     int flags = DP_TILE_NONTEMPORAL | DP_BATCH_WAVE_PER_TILE;
     uint32_t graph_steps = 32;
 
@@ -117,14 +120,27 @@ struct dp_batch {
         const uint32_t nx = sim->grid.extent[0], ny = sim->grid.extent[1], nz = sim->grid.extent[2];
         d.tiles_x = (nx + kAlignedX - 1) / kAlignedX;
         d.tiles_y = (ny + (rows - 2) - 1) / (rows - 2);
-        d.depth = std::min<uint32_t>(nz, depth);
+        d.depth = std::min<uint32_t>(nz, slab);
         d.tiles_z = (nz + d.depth - 1) / d.depth;
         d.tiles_total = d.tiles_x * d.tiles_y * d.tiles_z;
+    }
+
+    // The slab depth for depth 0: the single-sim rule applied to the x-y
+    // tiles of every member together, at most 16 (the measured best for a
+    // full batch).
+    uint32_t auto_depth() const {
+        uint64_t tiles_xy = 0;
+        for (const dp_sim *s : members)
+            tiles_xy += uint64_t((s->grid.extent[0] + kAlignedX - 1) / kAlignedX) *
+                        ((s->grid.extent[1] + (rows - 2) - 1) / (rows - 2));
+        const uint32_t capped = static_cast<uint32_t>(std::min<uint64_t>(tiles_xy, 0xFFFFFFFFull));
+        return std::min<uint32_t>(16, dp_sim::default_depth(capped));
     }
 
     // Instance table and tile owners for the current membership.
     void rebuild() {
         sync();
+        slab = depth ? depth : auto_depth();
         std::vector<InstanceDesc> table(members.size());
         std::vector<uint32_t> owner;
         table_intensity.assign(members.size(), nullptr);
@@ -454,7 +470,7 @@ int dp_batch_set_tile(dp_batch *batch, int rows, int depth, int flags) {
         if (depth < 0) fail(DP_ERR_INVALID, "depth must be non-negative");
         batch->sync();
         batch->rows = rows ? rows : 4;
-        batch->depth = depth ? static_cast<uint32_t>(depth) : 16;
+        batch->depth = static_cast<uint32_t>(depth);
         if (flags >= 0) batch->flags = flags & (DP_TILE_NONTEMPORAL | DP_TILE_BUFFER | DP_BATCH_WAVE_PER_TILE);
         batch->dirty = true;
     });
@@ -464,7 +480,7 @@ int dp_batch_tiling(const dp_batch *batch, int *rows, int *depth, int *flags) {
     return guarded([&] {
         if (!batch) fail(DP_ERR_INVALID, "null batch");
         if (rows) *rows = batch->rows;
-        if (depth) *depth = static_cast<int>(batch->depth);
+        if (depth) *depth = static_cast<int>(batch->depth ? batch->depth : batch->auto_depth());
         if (flags) *flags = batch->flags;
     });
 }

@@ -431,7 +431,7 @@ int dp_sim_autotune(dp_sim *sim, uint32_t trials, int *rows, int *depth, int *fl
         if (sim->kernel != DP_KERNEL_FUSED) fail(DP_ERR_UNSUPPORTED, "the autotuner tunes the aligned fused kernel");
         if (trials == 0 || trials > 1000) fail(DP_ERR_INVALID, "trials must be in 1..1000");
         static const int row_choices[] = {4, 6, 8};
-        static const uint32_t depth_choices[] = {16, 32, 64};
+        static const uint32_t depth_choices[] = {1, 2, 4, 8, 16, 32, 64};
         const bool buffer_fits = buffer_addressing_fits(sim->cells, sim->grid.extent[0], sim->grid.extent[1]);
         const int flag_choices[] = {DP_TILE_NONTEMPORAL, DP_TILE_NONTEMPORAL | DP_TILE_BUFFER};
         // Trial launches read the current half of each ping-pong pair and
@@ -456,6 +456,7 @@ int dp_sim_autotune(dp_sim *sim, uint32_t trials, int *rows, int *depth, int *fl
             if ((f & DP_TILE_BUFFER) && !buffer_fits) continue;
             for (int r : row_choices)
                 for (uint32_t d : depth_choices) {
+                    if (d > 2 * sim->grid.extent[2]) continue;  // the same slab as a shorter depth
                     trial.rows = r;
                     trial.depth = d;
                     trial.flags = f;

@@ -8,6 +8,7 @@
 // count, so requests that arrive while others run join the next launch
 // instead of waiting for the device to drain. Finished sims stay resident
 // until the client releases them.
+#include <algorithm>
 #include <condition_variable>
 #include <deque>
 #include <mutex>
@@ -51,8 +52,12 @@ struct Request {
     std::string error;
     dp_sim *sim = nullptr;
     uint64_t step = 0;  // progress as of the last finished chunk
+    uint32_t waited = 0;  // consecutive chunks this request sat out
     double submitted = 0, admitted = 0, started = 0, completed = 0;
 };
+
+// Chunks a resident request may sit out in a row before it steps anyway.
+constexpr uint32_t kMaxWaitChunks = 64;
 
 // Device bytes a sim of `cells` cells holds: two E and two H buffers of
 // 12 bytes per cell (ping-pong), the u8 material index, and a small fixed
@@ -193,7 +198,8 @@ struct dp_server {
                     running.push_back(r);
                 }
                 if (!running.empty()) {
-                    status = dp_batch_advance(batch, config.chunk_steps);
+                    status = schedule_chunk();
+                    if (status == DP_OK) status = dp_batch_advance(batch, config.chunk_steps);
                     if (status == DP_OK) status = dp_batch_wait(batch, 0);
                 }
             }
@@ -217,6 +223,31 @@ struct dp_server {
             running.swap(still);
             running_count = static_cast<uint32_t>(running.size());
         }
+    }
+
+    // Which members step this chunk: least remaining work (cells × steps
+    // left) first, until config.chunk_cells cells are stepping, plus any
+    // member that has waited kMaxWaitChunks chunks in a row. The others get
+    // their current step as target, so they sit the chunk out. Each member
+    // still advances exactly as it would alone.
+    int schedule_chunk() {
+        std::vector<Request *> order(running);
+        auto work = [](const Request *r) { return r->cells * (r->steps - r->sim->step); };
+        std::stable_sort(order.begin(), order.end(),
+                         [&](const Request *a, const Request *b) { return work(a) < work(b); });
+        uint64_t stepping = 0;
+        for (Request *r : order) {
+            const bool in = stepping < config.chunk_cells || r->waited >= kMaxWaitChunks;
+            if (in) {
+                stepping += r->cells;
+                r->waited = 0;
+            } else {
+                ++r->waited;
+            }
+            const int status = dp_batch_set_target(batch, r->sim, in ? r->steps : r->sim->step);
+            if (status != DP_OK) return status;
+        }
+        return DP_OK;
     }
 
     Request *find_locked(uint64_t ticket) {
@@ -255,6 +286,7 @@ int dp_server_create(dp_engine *engine, const dp_server_config *config, dp_serve
         if (c.chunk_steps == 0) c.chunk_steps = 32;
         if (c.host_threads == 0) c.host_threads = 4;
         if (c.host_threads > 64) fail(DP_ERR_INVALID, "at most 64 host threads");
+        if (c.chunk_cells == 0) c.chunk_cells = 32ull << 20;
         server->config = c;
         if (!c.one_at_a_time) {
             if (dp_batch_create(engine, &server->batch) != DP_OK) throw Error{DP_ERR_DEVICE, last_error()};

@@ -39,13 +39,18 @@ class RequestInfo:
 
 class Server:
     """A running request loop. ``one_at_a_time`` runs each request alone,
-    in arrival order, through the plain single-sim path (the baseline)."""
+    in arrival order, through the plain single-sim path (the baseline).
+    Otherwise resident requests share one batch; each chunk steps them in
+    order of least remaining work until ``chunk_cells`` cells are stepping
+    (0: 32M; ``ALL_MEMBERS``: every member every chunk)."""
+
+    ALL_MEMBERS = (1 << 64) - 1
 
     def __init__(self, context: Context, hbm_budget_bytes: int = 200 << 30, max_resident: int = 1024,
                  chunk_steps: int = 32, host_threads: int = 4, one_at_a_time: bool = False, rows: int = 0,
-                 depth: int = 0):
+                 depth: int = 0, chunk_cells: int = 0):
         config = DpServerConfig(hbm_budget_bytes, max_resident, chunk_steps, host_threads, int(one_at_a_time),
-                                rows, depth)
+                                rows, depth, chunk_cells)
         handle = c_void_p()
         check(lib().dp_server_create(context.handle, byref(config), byref(handle)))
         self._handle = handle

@@ -286,11 +286,15 @@ int dp_sim_set_kernel(dp_sim *sim, int kernel);
  *   DP_TILE_BUFFER        addresses the fields through buffer resources with
  *                         32-bit lane offsets (used when the grid's offsets
  *                         fit).
- * 0 for rows/depth selects the tuned default; flags < 0 keeps the current
- * flags. Results never depend on any of it. */
+ * 0 for rows selects 4; 0 for depth selects a size-aware default (the power
+ * of two at or below the number of x-y tiles / 50, within 1..32: short slabs
+ * give a small grid enough waves, long ones amortize a large grid's halo
+ * planes). flags < 0 keeps the current flags. Results never depend on any of
+ * it. */
 int dp_sim_set_tile(dp_sim *sim, int rows, int depth, int flags);
 /* Autotuner for DP_KERNEL_FUSED: times `trials` launches of every rows
- * (4, 6, 8) × depth (16, 32, 64) × addressing (pointer, buffer where it
+ * (4, 6, 8) × depth (1 to 64 in powers of two, up to twice the grid's z
+ * extent) × addressing (pointer, buffer where it
  * fits) candidate on this sim's grid, keeps the fastest and reports it (any
  * out pointer may be null). A sim keeps one tiling for plain steps and one
  * for intensity-accumulating steps; the autotuner sets the one for the
@@ -354,7 +358,9 @@ int dp_batch_wait(dp_batch *batch, uint32_t timeout_ms);
 /* Launches per captured HIP graph (default 32; 0 launches each kernel
  * directly). Results never depend on it. */
 int dp_batch_set_graph_steps(dp_batch *batch, uint32_t steps);
-/* Rows per wave (4, 6 or 8; 0 = default 4), z-slab depth (0 = 16) and
+/* Rows per wave (4, 6 or 8; 0 = default 4), z-slab depth (0, the default:
+ * sized to the membership, the single-sim rule over all members' x-y tiles
+ * together, at most 16) and
  * flags (< 0 keeps the current ones): DP_TILE_NONTEMPORAL, DP_TILE_BUFFER
  * (used when every member's offsets fit), and DP_BATCH_WAVE_PER_TILE, which
  * launches one wave per tile instead of the persistent grid whose waves
@@ -398,6 +404,13 @@ typedef struct dp_server_config {
     int32_t one_at_a_time;     /* 1: run requests alone, in arrival order,
                                   through dp_sim_advance (the baseline)   */
     int32_t rows, depth;       /* batch tiling (0: defaults)              */
+    uint64_t chunk_cells;      /* batched mode: members step in order of
+                                  least remaining work (cells × steps
+                                  left) until this many cells step in a
+                                  chunk; the others wait that chunk, and
+                                  none waits more than 64 chunks in a row.
+                                  0: 32M cells; UINT64_MAX: every member
+                                  steps every chunk                       */
 } dp_server_config;
 enum { DP_REQUEST_QUEUED = 0, DP_REQUEST_PREPARING = 1, DP_REQUEST_READY = 2, DP_REQUEST_RUNNING = 3,
        DP_REQUEST_DONE = 4 };

@@ -264,7 +264,8 @@ def test_autotuning_leaves_the_fields_unchanged(context):
     tuned = Simulation(context, scene)
     tuned.advance_by(50)
     choice = tuned.autotune(2)
-    assert choice["rows"] in (4, 6, 8) and choice["depth"] in (16, 32, 64) and choice["ms_per_step"] > 0.0
+    assert choice["rows"] in (4, 6, 8) and choice["depth"] in (1, 2, 4, 8, 16, 32, 64)
+    assert choice["ms_per_step"] > 0.0
     tuned.advance_by(30)
     tuned.set_accumulate_intensity(True)
     tuned.advance_by(10)

@@ -145,3 +145,29 @@ def test_a_failing_request_reports_its_error(context):
         server.release(bad)
         server.release(good)
         assert server.stats()["used_bytes"] == 0
+
+
+@pytest.mark.parametrize("chunk_cells", [1, Server.ALL_MEMBERS])
+def test_the_chunk_schedule_does_not_change_results(context, chunk_cells):
+    # chunk_cells = 1: one member (the least remaining work, or one that has
+    # waited too long) steps per chunk; ALL_MEMBERS: every member steps.
+    stream = small_stream(6, seed=26)
+    alone = [solo_fields(context, r) for r in stream]
+    with Server(context, chunk_steps=8, chunk_cells=chunk_cells) as server:
+        tickets = [server.submit(r.scene, r.steps) for r in stream]
+        for ticket in tickets:
+            server.wait(ticket)
+        completed = []
+        for k, (request, ticket) in enumerate(zip(stream, tickets)):
+            info = server.info(ticket)
+            assert info.state == DONE and info.step == request.steps
+            completed.append((info.completed_s, request.scene.grid.extent.total() * request.steps))
+            result = server.result(ticket)
+            assert bits_equal(result.read_electric(), alone[k][0]), f"request {k}: E differs"
+            assert bits_equal(result.read_magnetic(), alone[k][1]), f"request {k}: H differs"
+            server.release(ticket)
+    if chunk_cells == 1:
+        # least work first: the smallest request finishes no later than the largest
+        smallest = min(completed, key=lambda c: c[1])
+        largest = max(completed, key=lambda c: c[1])
+        assert smallest[0] <= largest[0]
