@@ -418,25 +418,47 @@ def main() -> int:
         sim.reset()
         sim.wait()
 
-    # Timed region: the workload's own step sequence from step 0.
+    # Timed region: the workload's own step sequence from step 0, with its
+    # readbacks of E to the host where it has them.
+    readback = work.readback_every if args.steps is None else None
     phase_a = accumulate_from if accumulate_from is not None else steps
     phase_b = steps - phase_a
-    sim.wait()
-    t0 = time.perf_counter()
-    sim.record_event(0)
-    sim.advance_by(phase_a)
-    sim.record_event(1)
-    if phase_b:
-        sim.set_accumulate_intensity(True)
-        sim.record_event(2)
-        sim.advance_by(phase_b)
-    sim.record_event(3)
-    sim.wait()
-    t1 = time.perf_counter()
-    wall = t1 - t0
 
-    ms_a = sim.elapsed_ms(0, 1)
-    ms_b = sim.elapsed_ms(2, 3) if phase_b else 0.0
+    def run(count: int) -> None:
+        if not readback:
+            sim.advance_by(count)
+            return
+        done = 0
+        while done < count:
+            chunk = min(readback, count - done)
+            sim.advance_by(chunk)
+            sim.read_electric()
+            done += chunk
+
+    def timed_pass(with_readback: bool):
+        nonlocal readback
+        saved, readback = readback, (readback if with_readback else None)
+        sim.wait()
+        start = time.perf_counter()
+        sim.record_event(0)
+        run(phase_a)
+        sim.record_event(1)
+        if phase_b:
+            sim.set_accumulate_intensity(True)
+            sim.record_event(2)
+            run(phase_b)
+        sim.record_event(3)
+        sim.wait()
+        elapsed = time.perf_counter() - start
+        readback = saved
+        return elapsed, sim.elapsed_ms(0, 1), sim.elapsed_ms(2, 3) if phase_b else 0.0
+
+    wall, ms_a, ms_b = timed_pass(True)
+    if readback:
+        # The kernels' own time, from the same sequence without the readbacks.
+        sim.reset()
+        sim.set_accumulate_intensity(False)
+        _, ms_a, ms_b = timed_pass(False)
     bytes_a = BYTES_PER_CELL * cells * phase_a
     bytes_b = BYTES_PER_CELL_INTENSITY * cells * phase_b
     kernel_s = (ms_a + ms_b) / 1e3
@@ -469,6 +491,7 @@ def main() -> int:
             "cells": cells,
             "extent": list(sim.extent),
             "steps_with_intensity": phase_b,
+            "readback_every": readback,
             "tiling": tiling,
             "note": work.note,
         },
