@@ -160,8 +160,8 @@ def bench_batched(args) -> int:
     for sim in sims:
         batch.add(sim, steps)
     if args.rows:
-        batch.set_tile(args.rows, args.depth)
-        tuned = {"rows": args.rows, "depth": args.depth or 32, "ms_per_step": None}
+        batch.set_tile(args.rows, args.depth, args.tile_flags)
+        tuned = dict(batch.tiling(), ms_per_step=None)
     else:
         rows, depth, ms = batch.autotune(3)
         tuned = dict(batch.tiling(), ms_per_step=ms)
@@ -356,8 +356,8 @@ def main() -> int:
     parser.add_argument("--rows", type=int, default=0, help="fused-kernel tile rows (0 = default / autotune)")
     parser.add_argument("--depth", type=int, default=0, help="fused-kernel z-slab depth (0 = default)")
     parser.add_argument("--tile-flags", type=int, default=-1,
-                        help="single-solve tile flags (1 XCD remap, 2 non-temporal stores, 4 buffer addressing; "
-                             "-1 = default)")
+                        help="tile flags (1 XCD remap, 2 non-temporal stores, 4 buffer addressing; batched: "
+                             "16 one wave per tile; -1 = default)")
     parser.add_argument("--no-cpu", action="store_true", help="skip the CPU-oracle baseline")
     parser.add_argument("--cpu-seconds", type=float, default=10.0)
     parser.add_argument("--requests", type=int, default=200, help="served_stream: number of requests")

@@ -33,6 +33,24 @@ def counter_values(path: str, kernel: str, counter: str) -> List[float]:
             if _match(r["Kernel_Name"], kernel) and r["Counter_Name"] == counter]
 
 
+def display_name(traced: str) -> str:
+    """A traced kernel name without its return type, namespaces and argument list."""
+    name = traced.strip()
+    if name.startswith("void "):
+        name = name[len("void "):]
+    depth = 0
+    for i, ch in enumerate(name):  # cut at the argument list's opening parenthesis
+        if ch == "<":
+            depth += 1
+        elif ch == ">":
+            depth -= 1
+        elif ch == "(" and depth == 0 and i > 0 and not name[:i].endswith("::") and name[i - 1] != "(":
+            if not name.startswith("anonymous", i + 1):
+                name = name[:i]
+                break
+    return name.replace("dpa::(anonymous namespace)::", "").replace("dpa::", "")
+
+
 def main(argv: Optional[List[str]] = None) -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--kernel", required=True)
@@ -63,8 +81,9 @@ def main(argv: Optional[List[str]] = None) -> int:
     algorithmic = args.cells * args.bytes_per_cell
     achieved = algorithmic / (avg_ms / 1e3) / 1e9
 
+    kernel_name = display_name(stats[0]["Name"]) if len({r["Name"] for r in stats}) == 1 else args.kernel
     summary = {
-        "kernel": args.kernel,
+        "kernel": kernel_name,
         "launches": calls,
         "avg_ms": avg_ms,
         "algorithmic_bytes_per_launch": algorithmic,
@@ -72,7 +91,7 @@ def main(argv: Optional[List[str]] = None) -> int:
         "peak_gbs": args.peak_gbs,
         "frac": achieved / args.peak_gbs,
     }
-    lines = [f"# rocprofv3 summary: {args.kernel}", "", f"Command: `{args.command}`", "",
+    lines = [f"# rocprofv3 summary: {kernel_name}", "", f"Command: `{args.command}`", "",
              "| quantity | value |", "|---|---|",
              f"| launches profiled | {calls} |",
              f"| average duration | {avg_ms:.3f} ms |",
