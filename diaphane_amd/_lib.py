@@ -61,6 +61,10 @@ def _declare(lib: ctypes.CDLL) -> None:
         "dp_grid_resolve": (c_int, [POINTER(DpGridSpec), POINTER(c_uint32), POINTER(c_float), c_size_t,
                                     POINTER(c_float)]),
         "dp_scene_validate": (c_int, [POINTER(DpScene)]),
+        "dp_scene_from_ron": (c_int, [c_char_p, POINTER(c_void_p)]),
+        "dp_scene_file_scene": (POINTER(DpScene), [c_void_p]),
+        "dp_scene_file_destroy": (None, [c_void_p]),
+        "dp_scene_to_ron": (c_int, [POINTER(DpScene), c_char_p, c_size_t, POINTER(c_size_t)]),
         "dp_scene_material_indices": (c_int, [POINTER(DpScene), POINTER(c_uint32), c_size_t]),
         "dp_scene_injections": (c_int, [POINTER(DpScene), c_float, POINTER(DpInjection), c_size_t]),
         "dp_scene_tables": (c_int, [POINTER(DpScene), POINTER(c_float), c_size_t, POINTER(c_float), c_size_t]),

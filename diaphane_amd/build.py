@@ -35,6 +35,7 @@ HOST = ["-x", "c++", "-D__HIP_PLATFORM_AMD__", f"-I{ROCM}/include"]
 SOURCES = [
     ("kernels.hip", DEVICE),
     ("scene.cpp", HOST),
+    ("scene_ron.cpp", HOST),
     ("engine.cpp", HOST),
     ("batch.cpp", HOST),
     ("server.cpp", HOST),

@@ -2,6 +2,8 @@
 // (src/gpu.rs:159-799), built on HIP streams and hand-written gfx950 kernels.
 #include "engine.hpp"
 
+#include <cstring>
+
 #include <chrono>
 #include <limits>
 #include <thread>
@@ -49,6 +51,35 @@ int dp_grid_resolve(const dp_grid_spec *spec, uint32_t extent_out[3], float *wid
                 for (float w : g.axis[a].primary) widths_out[at++] = w;
         }
         if (time_step_out) *time_step_out = g.time_step();
+    });
+}
+
+struct dp_scene_file {
+    std::unique_ptr<dpa::SceneFile> file;
+};
+
+int dp_scene_from_ron(const char *text, dp_scene_file **out) {
+    return guarded([&] {
+        if (!text || !out) fail(DP_ERR_INVALID, "null argument");
+        auto owned = std::make_unique<dp_scene_file>();
+        owned->file = parse_scene_ron(text);
+        *out = owned.release();
+    });
+}
+
+const dp_scene *dp_scene_file_scene(const dp_scene_file *file) { return file ? &file->file->scene : nullptr; }
+
+void dp_scene_file_destroy(dp_scene_file *file) { delete file; }
+
+int dp_scene_to_ron(const dp_scene *scene, char *out, size_t capacity, size_t *length) {
+    return guarded([&] {
+        check_scene_pointers(scene);
+        if (!length) fail(DP_ERR_INVALID, "null length");
+        const std::string text = scene_to_ron(*scene);
+        *length = text.size();
+        if (!out) return;
+        if (capacity < text.size() + 1) fail(DP_ERR_SIZE, "the buffer holds fewer bytes than the text and its NUL");
+        std::memcpy(out, text.c_str(), text.size() + 1);
     });
 }
 

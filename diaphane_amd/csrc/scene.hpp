@@ -9,6 +9,7 @@
 #pragma once
 
 #include <cstdint>
+#include <memory>
 #include <string>
 #include <vector>
 
@@ -92,6 +93,20 @@ void rasterize_u8(const dp_scene &scene, const Grid &grid, uint8_t *out);
 
 // Scene::validate (src/scene.rs:254-369); empty string when valid.
 std::string validate(const dp_scene &scene);
+
+// A scene read from RON text, owning the arrays its dp_scene points into.
+struct SceneFile {
+    dp_scene scene{};
+    std::vector<dp_refinement> refinements;
+    std::vector<dp_material> materials;
+    std::vector<dp_shape> shapes;
+    std::vector<dp_source> sources;
+    void bind();  // points `scene` at the vectors
+};
+
+// Scene::from_ron / to_ron (src/scene.rs:467-498): the RON text format.
+std::unique_ptr<SceneFile> parse_scene_ron(const std::string &text);
+std::string scene_to_ron(const dp_scene &scene);
 bool scene_is_reversible(const dp_scene &scene);
 
 // Thread-local last error, shared by every C entry point.

@@ -173,6 +173,11 @@ class Refinement:
     size: Tuple[float, float, float]
     cell_size: float
 
+    def __post_init__(self):
+        object.__setattr__(self, "center", tuple(float(f32(v)) for v in self.center))
+        object.__setattr__(self, "size", tuple(float(f32(v)) for v in self.size))
+        object.__setattr__(self, "cell_size", float(f32(self.cell_size)))
+
     @staticmethod
     def across(axis: Axis, offset: float, size: float, cell_size: float) -> "Refinement":
         center = [0.0, 0.0, 0.0]
@@ -213,6 +218,15 @@ class Grid:
         self.max_ratio = float(f32(max_ratio))
         self._courant = float(f32(courant))
         self._resolved = None
+
+    def _recipe(self):
+        return (self.base_extent, self.base_cell_size, self._courant, tuple(self.refinements), self.max_ratio)
+
+    def __eq__(self, other) -> bool:
+        """Grids are equal when their recipes are (the resolved spacing follows)."""
+        return isinstance(other, Grid) and self._recipe() == other._recipe()
+
+    __hash__ = None
 
     # constructors (src/grid.rs:415-456)
     @staticmethod
@@ -397,6 +411,11 @@ class MaterialTable:
 
     def copy(self) -> "MaterialTable":
         return MaterialTable(self.materials)
+
+    def __eq__(self, other) -> bool:
+        return isinstance(other, MaterialTable) and self.materials == other.materials
+
+    __hash__ = None
 
 
 # ---------------------------------------------------------------------------
@@ -588,6 +607,75 @@ class Scene:
     @staticmethod
     def on_grid(grid: Grid) -> "Scene":
         return Scene(grid)
+
+    def with_resolution(self, resolution: float) -> "Scene":
+        """Scene::with_resolution (src/scene.rs:158-163): the same scene on a
+        grid of `resolution` cells per metre."""
+        return Scene(self.grid.with_resolution(resolution), self.boundary, self.materials.copy(),
+                     list(self.shapes), list(self.sources))
+
+    # scene files (src/scene.rs:467-498)
+    @staticmethod
+    def from_ron(text: str) -> "Scene":
+        """Scene::from_ron: parses the RON text format (not validated).
+        Raises ValueError with the line and column of a syntax error."""
+        from ._lib import DiaphaneError, lib
+
+        handle = ctypes.c_void_p()
+        status = lib().dp_scene_from_ron(text.encode("utf-8"), ctypes.byref(handle))
+        if status != 0:
+            from ._lib import last_error
+            raise ValueError(last_error())
+        try:
+            return Scene.from_c(lib().dp_scene_file_scene(handle).contents)
+        finally:
+            lib().dp_scene_file_destroy(handle)
+
+    def to_ron(self) -> str:
+        """Scene::to_ron: text that from_ron reads back to an equal scene."""
+        from ._lib import check, lib
+
+        c = self.to_c()
+        length = ctypes.c_size_t()
+        check(lib().dp_scene_to_ron(ctypes.byref(c), None, 0, ctypes.byref(length)))
+        buffer = ctypes.create_string_buffer(length.value + 1)
+        check(lib().dp_scene_to_ron(ctypes.byref(c), buffer, len(buffer), ctypes.byref(length)))
+        return buffer.value.decode("utf-8")
+
+    @staticmethod
+    def load(path) -> "Scene":
+        """Scene::load: reads a scene file; errors name the file."""
+        with open(path, encoding="utf-8") as handle:
+            text = handle.read()
+        try:
+            return Scene.from_ron(text)
+        except ValueError as error:
+            raise ValueError(f"{path}: {error}") from None
+
+    def save(self, path) -> None:
+        with open(path, "w", encoding="utf-8") as handle:
+            handle.write(self.to_ron())
+
+    @staticmethod
+    def from_c(c: "DpScene") -> "Scene":
+        """A Python scene holding the values of a C dp_scene."""
+        g = c.grid
+        refinements = [Refinement(tuple(g.refinements[k].center), tuple(g.refinements[k].size),
+                                  g.refinements[k].cell_size) for k in range(g.refinement_count)]
+        grid = Grid(Extent(*g.extent), g.cell_size, g.courant, refinements, g.max_ratio)
+        boundary = Boundary.PEC if c.boundary == 0 else Boundary.absorbing(c.absorber_thickness,
+                                                                           c.target_reflection)
+        materials = MaterialTable([Material(m.relative_permittivity, m.relative_permeability, m.conductivity,
+                                            m.magnetic_loss) for m in c.materials[:c.material_count]])
+        shapes = [Shape(s.kind, s.material, tuple(s.center), tuple(s.size), s.radius, Axis(s.axis), s.offset,
+                        s.thickness) for s in c.shapes[:c.shape_count]]
+        sources = []
+        for s in c.sources[:c.source_count]:
+            w = s.waveform
+            sources.append(Source(s.shape, Axis(s.polarization), Waveform(w.kind, w.frequency, w.cycles, w.delay,
+                                                                          w.width),
+                                  s.amplitude, tuple(s.at), tuple(s.velocity), Axis(s.axis), s.offset, s.waist))
+        return Scene(grid, boundary, materials, shapes, sources)
 
     def with_boundary(self, boundary: Boundary) -> "Scene":
         self.boundary = boundary

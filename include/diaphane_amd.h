@@ -142,6 +142,22 @@ typedef struct dp_scene {
 /* Host-side scene resolution (no device needed).                           */
 /* ------------------------------------------------------------------------ */
 
+/* Scene files in the reference's RON format: Scene::from_ron and to_ron
+ * (src/scene.rs:467-498; the GridSpec checks of src/grid.rs:894-935 come
+ * back as errors). dp_scene_from_ron parses `text` (NUL-terminated) into a
+ * scene the returned file owns; a syntax error names its line and column.
+ * The scene is not validated: call dp_scene_validate. */
+typedef struct dp_scene_file dp_scene_file;
+int dp_scene_from_ron(const char *text, dp_scene_file **out);
+/* The parsed scene, valid until dp_scene_file_destroy. */
+const dp_scene *dp_scene_file_scene(const dp_scene_file *file);
+void dp_scene_file_destroy(dp_scene_file *file);
+/* Writes `scene` as RON that dp_scene_from_ron reads back to the same
+ * values. *length gets the text's length (without the NUL); with out ==
+ * NULL nothing is written, otherwise capacity must hold text and NUL
+ * (DP_ERR_SIZE if not). */
+int dp_scene_to_ron(const dp_scene *scene, char *out, size_t capacity, size_t *length);
+
 /* Message of the last failed call on this thread ("" if none). */
 const char *dp_last_error(void);
 
