@@ -101,6 +101,12 @@ def _declare(lib: ctypes.CDLL) -> None:
         "dp_sim_launch_count": (c_uint64, [sim]),
         "dp_sim_stream_probe": (c_int, [sim, c_int, POINTER(c_double)]),
         "dp_sim_tile_probe": (c_int, [sim, c_int, c_int, c_int, POINTER(c_double)]),
+        "dp_timeline_create": (c_int, [c_uint64, c_uint32, POINTER(c_void_p)]),
+        "dp_timeline_destroy": (None, [c_void_p]),
+        "dp_timeline_observe": (c_int, [c_void_p, sim]),
+        "dp_timeline_seek": (c_int, [c_void_p, sim, c_uint64, POINTER(c_int), POINTER(c_uint64), POINTER(c_uint64)]),
+        "dp_timeline_clear": (c_int, [c_void_p]),
+        "dp_timeline_info": (c_int, [c_void_p, POINTER(c_uint32), POINTER(c_uint64), POINTER(c_uint64)]),
         "dp_batch_create": (c_int, [c_void_p, POINTER(c_void_p)]),
         "dp_batch_destroy": (None, [c_void_p]),
         "dp_batch_add": (c_int, [c_void_p, sim]),

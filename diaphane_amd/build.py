@@ -36,6 +36,7 @@ SOURCES = [
     ("kernels.hip", DEVICE),
     ("scene.cpp", HOST),
     ("scene_ron.cpp", HOST),
+    ("timeline.cpp", HOST),
     ("engine.cpp", HOST),
     ("batch.cpp", HOST),
     ("server.cpp", HOST),

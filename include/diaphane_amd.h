@@ -340,6 +340,31 @@ int dp_sim_stream_probe(dp_sim *sim, int iterations, double *gbs);
  * memory traffic. No counterpart in the reference: a measurement aid. */
 int dp_sim_tile_probe(dp_sim *sim, int rows, int depth, int iterations, double *gbs);
 
+/* Keyframes in device memory: Timeline (src/timeline.rs:76-212) whose
+ * snapshots (E and H, 24 bytes per cell) stay in HBM. A ring of `capacity`
+ * keyframes, one per `interval` steps (Timeline::new, :113-125); the grid
+ * size is fixed by the first keyframe (DP_ERR_SIZE for another grid later).
+ * Recording and restoring are device-to-device copies on the sim's stream.
+ * Not for batch members (they are advanced by their batch). */
+typedef struct dp_timeline dp_timeline;
+enum { DP_SEEK_UNCHANGED = 0, DP_SEEK_FORWARD = 1, DP_SEEK_FROM_KEYFRAME = 2, DP_SEEK_FROM_START = 3 };
+int dp_timeline_create(uint64_t interval, uint32_t capacity, dp_timeline **out);
+void dp_timeline_destroy(dp_timeline *timeline);
+/* Timeline::observe (:127-148): records the sim's state if it has crossed
+ * into a new interval past the newest keyframe. */
+int dp_timeline_observe(dp_timeline *timeline, dp_sim *sim);
+/* Timeline::seek (:167-211): forward by stepping; backward by restoring the
+ * newest keyframe at or before `target` (which clears the intensity sum, as
+ * a restore does) and replaying, or from a reset when none is. kind gets
+ * DP_SEEK_*, replayed the steps replayed, at the keyframe's step (each out
+ * pointer may be null). */
+int dp_timeline_seek(dp_timeline *timeline, dp_sim *sim, uint64_t target, int *kind, uint64_t *replayed,
+                     uint64_t *at);
+/* Timeline::clear. */
+int dp_timeline_clear(dp_timeline *timeline);
+/* keyframe_count, earliest (0 when there is none) and bytes (:150-165). */
+int dp_timeline_info(const dp_timeline *timeline, uint32_t *keyframes, uint64_t *earliest, uint64_t *bytes);
+
 /* Forward-step launches enqueued so far (fused or H-kernel launches). */
 uint64_t dp_sim_launch_count(const dp_sim *sim);
 
