@@ -418,6 +418,26 @@ int dp_sim_energy(dp_sim *sim, double *electric, double *magnetic) {
     });
 }
 
+int dp_sim_measure_peak(dp_sim *sim, float peak[8]) {
+    return guarded([&] {
+        if (!sim || !peak) fail(DP_ERR_INVALID, "null argument");
+        const uint32_t blocks = peak_blocks(sim->cells);
+        const size_t bytes = size_t(blocks) * 8 * sizeof(float);
+        if (sim->peak_partials.bytes < bytes) sim->peak_partials.allocate(bytes);
+        const double *intensity = sim->intensity.ptr && sim->accumulated ? sim->intensity.as<double>() : nullptr;
+        launch_measure_peak(sim->e[sim->current].as<float>(), sim->h[sim->current].as<float>(), intensity,
+                            sim->accumulated, sim->cells, sim->peak_partials.as<float>(), sim->stream);
+        check(hipGetLastError(), "kernel launch");
+        std::vector<float> partials(size_t(blocks) * 8);
+        copy_on(sim->stream, partials.data(), sim->peak_partials.ptr, bytes, hipMemcpyDeviceToHost);
+        for (int k = 0; k < 8; ++k) {
+            float m = 0.0f;
+            for (uint32_t b = 0; b < blocks; ++b) m = std::max(m, partials[size_t(b) * 8 + k]);
+            peak[k] = m;
+        }
+    });
+}
+
 int dp_sim_device_ptr(dp_sim *sim, int which, void **out) {
     return guarded([&] {
         if (!sim || !out) fail(DP_ERR_INVALID, "null argument");

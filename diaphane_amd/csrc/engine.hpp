@@ -124,6 +124,7 @@ struct dp_sim {
     DeviceBuffer e[2], h[2];
     DeviceBuffer material, coefficients, axis_table, weights, eps_mu, widths;
     DeviceBuffer intensity, injections, partials;
+    DeviceBuffer peak_partials;  // per-block maxima of dp_sim_measure_peak
     PinnedBuffer staging;
     int current = 0;  // which of the ping-pong pair holds the state
 

@@ -400,6 +400,49 @@ __global__ __launch_bounds__(256) void k_intensity(const float *e, double *inten
 // the result is deterministic.
 constexpr uint32_t kEnergyThreads = 256;
 
+// The viewer's auto-range reduction (examples/common/render.rs:382-419,
+// shaders/volume.wgsl:122-132, 376-378, 473-493), eight maxima per block:
+//   [0]    max over cells of max(½|E|², ½|H|²), each sum ((0+x²)+y²)+z² in f32
+//   [1..3] max |E_x|, |E_y|, |E_z|;  [4..6] max |H_x|, |H_y|, |H_z|
+//   [7]    max of the intensity sum (rounded to f32) / f32(steps), 0 if none.
+// Maxima are exact, so the result does not depend on the block split; the
+// host reduces the per-block partials.
+constexpr uint32_t kPeakThreads = 256;
+constexpr uint32_t kPeakSlots = 8;
+
+__global__ __launch_bounds__(kPeakThreads) void k_measure_peak(const float *e, const float *h,
+                                                               const double *intensity, float steps,
+                                                               uint64_t n, float *partials) {
+    float m[kPeakSlots] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    for (uint64_t i = uint64_t(blockIdx.x) * kPeakThreads + threadIdx.x; i < n;
+         i += uint64_t(gridDim.x) * kPeakThreads) {
+        float se = 0.f, sh = 0.f;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            const float ev = e[c * n + i], hv = h[c * n + i];
+            se = se + ev * ev;
+            sh = sh + hv * hv;
+            m[1 + c] = fmaxf(m[1 + c], fabsf(ev));
+            m[4 + c] = fmaxf(m[4 + c], fabsf(hv));
+        }
+        m[0] = fmaxf(m[0], fmaxf(0.5f * se, 0.5f * sh));
+        if (intensity) m[7] = fmaxf(m[7], float(intensity[i]) / steps);
+    }
+    __shared__ float s_peak[kPeakSlots][kPeakThreads];
+#pragma unroll
+    for (uint32_t k = 0; k < kPeakSlots; ++k) s_peak[k][threadIdx.x] = m[k];
+    __syncthreads();
+    for (uint32_t stride = kPeakThreads / 2; stride > 0; stride /= 2) {
+        if (threadIdx.x < stride) {
+#pragma unroll
+            for (uint32_t k = 0; k < kPeakSlots; ++k)
+                s_peak[k][threadIdx.x] = fmaxf(s_peak[k][threadIdx.x], s_peak[k][threadIdx.x + stride]);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x < kPeakSlots) partials[blockIdx.x * kPeakSlots + threadIdx.x] = s_peak[threadIdx.x][0];
+}
+
 __global__ __launch_bounds__(kEnergyThreads) void k_energy(const float *e, const float *h,
                                                            const uint8_t *material,
                                                            const float2 *eps_mu,
@@ -731,6 +774,18 @@ uint32_t energy_blocks(uint32_t nx, uint32_t ny, uint32_t nz) {
     const uint64_t n = uint64_t(nx) * ny * nz;
     const uint64_t wanted = (n + kEnergyThreads - 1) / kEnergyThreads;
     return uint32_t(wanted < 4096 ? wanted : 4096);
+}
+
+uint32_t peak_blocks(uint64_t cells) {
+    const uint64_t wanted = (cells + kPeakThreads * 16 - 1) / (kPeakThreads * 16);
+    return static_cast<uint32_t>(std::max<uint64_t>(1, std::min<uint64_t>(wanted, 4096)));
+}
+
+uint32_t launch_measure_peak(const float *e, const float *h, const double *intensity, uint64_t steps, uint64_t cells,
+                             float *partials, hipStream_t stream) {
+    const uint32_t blocks = peak_blocks(cells);
+    k_measure_peak<<<blocks, kPeakThreads, 0, stream>>>(e, h, intensity, float(steps ? steps : 1), cells, partials);
+    return blocks;
 }
 
 uint32_t launch_energy(const float *e, const float *h, const uint8_t *material,

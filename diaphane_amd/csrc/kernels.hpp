@@ -116,6 +116,12 @@ uint32_t launch_energy(const float *e, const float *h, const uint8_t *material,
                        const float2 *eps_mu, const float *widths, uint32_t nx, uint32_t ny,
                        uint32_t nz, double *partials, hipStream_t stream);
 uint32_t energy_blocks(uint32_t nx, uint32_t ny, uint32_t nz);
+// The viewer's auto-range maxima: per block eight floats into `partials`
+// (peak_blocks(cells) × 8); returns the number of blocks. `intensity` may be
+// null; `steps` is the accumulated step count (0 counts as 1).
+uint32_t peak_blocks(uint64_t cells);
+uint32_t launch_measure_peak(const float *e, const float *h, const double *intensity, uint64_t steps, uint64_t cells,
+                             float *partials, hipStream_t stream);
 // Streams the step's access mix (E, H, material in; E, H out) with no
 // stencil: the practical bandwidth ceiling for the step's byte count.
 void launch_stream_probe(const float *e, const float *h, const uint8_t *material, float *e_out, float *h_out,

@@ -71,8 +71,11 @@ struct dp_timeline {
         }
         if (s.read) check(hipStreamWaitEvent(sim->stream, s.used, 0), "hipStreamWaitEvent");
         char *base = static_cast<char *>(s.fields.ptr);
-        copy_on(sim->stream, base, sim->e[sim->current].ptr, field_bytes, hipMemcpyDeviceToDevice);
-        copy_on(sim->stream, base + field_bytes, sim->h[sim->current].ptr, field_bytes, hipMemcpyDeviceToDevice);
+        check(hipMemcpyAsync(base, sim->e[sim->current].ptr, field_bytes, hipMemcpyDeviceToDevice, sim->stream),
+              "hipMemcpyAsync");
+        check(hipMemcpyAsync(base + field_bytes, sim->h[sim->current].ptr, field_bytes, hipMemcpyDeviceToDevice,
+                             sim->stream),
+              "hipMemcpyAsync");
         check(hipEventRecord(s.ready, sim->stream), "hipEventRecord");
         s.filled = true;
         ring.push_back({index, sim->step});
@@ -84,8 +87,11 @@ struct dp_timeline {
         sim->clear_intensity();  // the average belongs to the abandoned history
         check(hipStreamWaitEvent(sim->stream, s.ready, 0), "hipStreamWaitEvent");
         const char *base = static_cast<const char *>(s.fields.ptr);
-        copy_on(sim->stream, sim->e[sim->current].ptr, base, field_bytes, hipMemcpyDeviceToDevice);
-        copy_on(sim->stream, sim->h[sim->current].ptr, base + field_bytes, field_bytes, hipMemcpyDeviceToDevice);
+        check(hipMemcpyAsync(sim->e[sim->current].ptr, base, field_bytes, hipMemcpyDeviceToDevice, sim->stream),
+              "hipMemcpyAsync");
+        check(hipMemcpyAsync(sim->h[sim->current].ptr, base + field_bytes, field_bytes, hipMemcpyDeviceToDevice,
+                             sim->stream),
+              "hipMemcpyAsync");
         check(hipEventRecord(s.used, sim->stream), "hipEventRecord");
         s.read = true;
         sim->step = frame.step;

@@ -242,6 +242,13 @@ class Simulation:
         check(lib().dp_sim_stream_probe(self._handle, iterations, byref(gbs)))
         return gbs.value
 
+    def measure_peak(self) -> np.ndarray:
+        """The viewer's auto-range maxima (8 floats): max(½|E|², ½|H|²), max
+        |E_x..z|, max |H_x..z|, and the peak averaged intensity."""
+        out = np.zeros(8, dtype=np.float32)
+        check(lib().dp_sim_measure_peak(self._handle, _fptr(out)))
+        return out
+
     def tile_probe(self, rows: int = 0, depth: int = 0, iterations: int = 10) -> float:
         """GB/s (49 B/cell) of the fused kernel's tile access pattern with no
         stencil, at `rows` × `depth` (0: the plain tiling's)."""

@@ -270,6 +270,16 @@ int dp_sim_read_intensity_f64(dp_sim *sim, double *out, size_t count);
 int dp_sim_write_fields(dp_sim *sim, const float *electric,
                         const float *magnetic, size_t count, uint64_t step);
 
+/* The viewer's auto-range reduction (measure_peak, examples/common/
+ * render.rs:382-419 with shaders/volume.wgsl:122-132, 376-378, 473-493), so
+ * a renderer that binds dp_sim_device_ptr buffers gets its exposure without
+ * a readback of the fields. peak[0]: max over cells of the larger of ½|E|²
+ * and ½|H|² (each sum ((0+x²)+y²)+z² in f32); peak[1..3]: max |E_x|, |E_y|,
+ * |E_z|; peak[4..6]: max |H_x|, |H_y|, |H_z|; peak[7]: max of the intensity
+ * sum (rounded to f32) over the accumulated step count, 0 with no sum.
+ * Synchronous. */
+int dp_sim_measure_peak(dp_sim *sim, float peak[8]);
+
 /* cpu::Simulation::energy (src/cpu.rs:654-687) computed on the device:
  * per-component volumes from the primary/dual widths, f64 accumulation. */
 int dp_sim_energy(dp_sim *sim, double *electric, double *magnetic);
