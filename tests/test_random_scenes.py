@@ -65,3 +65,13 @@ def test_the_draws_cover_every_kind():
     assert sources == {0, 1, 2} and waveforms == {0, 1, 2, 3} and shapes == {0, 1, 2}
     assert boundaries == {"pec", "absorbing"}
     assert graded > 0 and moving > 0 and ragged > 30
+
+
+@pytest.mark.parametrize("seed", SEEDS)
+def test_random_scenes_survive_a_ron_round_trip(seed):
+    # Every construct the draws use (graded grids, both boundaries, all
+    # shapes, sources and waveforms) written as RON and read back equal.
+    from diaphane_amd import Scene
+
+    case = random_case(seed)
+    assert Scene.from_ron(case.scene.to_ron()) == case.scene, case.describe()
