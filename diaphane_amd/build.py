@@ -10,6 +10,7 @@ import os
 import shutil
 import subprocess
 import sys
+from concurrent.futures import ThreadPoolExecutor
 from pathlib import Path
 
 HERE = Path(__file__).resolve().parent
@@ -34,6 +35,7 @@ HOST = ["-x", "c++", "-D__HIP_PLATFORM_AMD__", f"-I{ROCM}/include"]
 
 SOURCES = [
     ("kernels.hip", DEVICE),
+    ("kernels_pair.hip", DEVICE),
     ("scene.cpp", HOST),
     ("scene_ron.cpp", HOST),
     ("timeline.cpp", HOST),
@@ -53,15 +55,19 @@ def hipcc() -> str:
 def build(verbose: bool = False) -> Path:
     compiler = hipcc()
     BUILD.mkdir(exist_ok=True)
-    objects = []
+    commands, objects = [], []
     for name, flags in SOURCES:
         src = CSRC / name
         obj = BUILD / (src.stem + ".o")
-        cmd = [compiler, *flags, *COMMON, "-c", str(src), "-o", str(obj)]
-        if verbose:
-            print(" ".join(cmd), flush=True)
-        subprocess.run(cmd, check=True)
+        commands.append([compiler, *flags, *COMMON, "-c", str(src), "-o", str(obj)])
         objects.append(str(obj))
+    # The translation units compile in parallel (the two kernel units take
+    # minutes each; the host units seconds).
+    with ThreadPoolExecutor(max_workers=min(len(commands), os.cpu_count() or 1)) as pool:
+        for cmd in commands:
+            if verbose:
+                print(" ".join(cmd), flush=True)
+        list(pool.map(lambda cmd: subprocess.run(cmd, check=True), commands))
     tmp = LIBRARY.with_suffix(".so.tmp")
     cmd = [compiler, "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", str(tmp), *objects, "-lpthread"]
     if verbose:

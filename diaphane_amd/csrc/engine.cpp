@@ -470,7 +470,8 @@ int dp_sim_set_tile(dp_sim *sim, int rows, int depth, int flags) {
         for (auto &t : sim->tiling) {
             t.rows = rows;
             t.depth = static_cast<uint32_t>(depth);
-            if (flags >= 0) t.flags = flags & (DP_TILE_XCD_REMAP | DP_TILE_NONTEMPORAL | DP_TILE_BUFFER | 0xFF00);
+            if (flags >= 0)
+                t.flags = flags & (DP_TILE_XCD_REMAP | DP_TILE_NONTEMPORAL | DP_TILE_BUFFER | DP_TILE_TWO_STEP | 0xFF00);
         }
     });
 }
@@ -482,6 +483,7 @@ int dp_sim_autotune(dp_sim *sim, uint32_t trials, int *rows, int *depth, int *fl
         if (sim->kernel != DP_KERNEL_FUSED) fail(DP_ERR_UNSUPPORTED, "the autotuner tunes the aligned fused kernel");
         if (trials == 0 || trials > 1000) fail(DP_ERR_INVALID, "trials must be in 1..1000");
         static const int row_choices[] = {4, 6, 8};
+        static const int pair_row_choices[] = {6, 8, 10};
         static const uint32_t depth_choices[] = {1, 2, 4, 8, 16, 32, 64};
         const bool buffer_fits = buffer_addressing_fits(sim->cells, sim->grid.extent[0], sim->grid.extent[1]);
         const int flag_choices[] = {DP_TILE_NONTEMPORAL, DP_TILE_NONTEMPORAL | DP_TILE_BUFFER};
@@ -503,35 +505,44 @@ int dp_sim_autotune(dp_sim *sim, uint32_t trials, int *rows, int *depth, int *fl
         float best = std::numeric_limits<float>::infinity();
         int best_rows = 4, best_flags = DP_TILE_NONTEMPORAL;
         uint32_t best_depth = 32;
+        a.injections2 = a.injections;
+        a.n_magnetic2 = a.n_electric2 = 0;
+        auto try_shape = [&](int r, uint32_t d, int f) {
+            const bool two_step = (f & DP_TILE_TWO_STEP) != 0;
+            trial.rows = r;
+            trial.depth = d;
+            trial.flags = f;
+            sim->fill_tiles(a, r, two_step ? kPairOwnedX : kAlignedX, trial, two_step ? 4 : 2);
+            auto launch = [&] {
+                if (two_step)
+                    launch_fused_pair(a, r, (f & DP_TILE_NONTEMPORAL) != 0, (f & DP_TILE_BUFFER) != 0, sim->stream);
+                else
+                    launch_fused_aligned(a, r, (f & DP_TILE_NONTEMPORAL) != 0, (f & DP_TILE_BUFFER) != 0,
+                                         sim->stream);
+            };
+            launch();  // the first launch of a shape loads its code object
+            check(hipGetLastError(), "kernel launch");
+            check(hipEventRecord(sim->events[6], sim->stream), "hipEventRecord");
+            for (uint32_t t = 0; t < trials; ++t) launch();
+            check(hipEventRecord(sim->events[7], sim->stream), "hipEventRecord");
+            check(hipEventSynchronize(sim->events[7]), "hipEventSynchronize");
+            float ms = 0.0f;
+            check(hipEventElapsedTime(&ms, sim->events[6], sim->events[7]), "hipEventElapsedTime");
+            ms /= static_cast<float>(two_step ? 2 * trials : trials);  // per step
+            if (ms < best) {
+                best = ms;
+                best_rows = r;
+                best_depth = d;
+                best_flags = f;
+            }
+        };
         for (int f : flag_choices) {
             if ((f & DP_TILE_BUFFER) && !buffer_fits) continue;
-            for (int r : row_choices)
-                for (uint32_t d : depth_choices) {
-                    if (d > 2 * sim->grid.extent[2]) continue;  // the same slab as a shorter depth
-                    trial.rows = r;
-                    trial.depth = d;
-                    trial.flags = f;
-                    sim->fill_tiles(a, r, kAlignedX, trial);
-                    auto launch = [&] {
-                        launch_fused_aligned(a, r, (f & DP_TILE_NONTEMPORAL) != 0, (f & DP_TILE_BUFFER) != 0,
-                                             sim->stream);
-                    };
-                    launch();  // the first launch of a shape loads its code object
-                    check(hipGetLastError(), "kernel launch");
-                    check(hipEventRecord(sim->events[6], sim->stream), "hipEventRecord");
-                    for (uint32_t t = 0; t < trials; ++t) launch();
-                    check(hipEventRecord(sim->events[7], sim->stream), "hipEventRecord");
-                    check(hipEventSynchronize(sim->events[7]), "hipEventSynchronize");
-                    float ms = 0.0f;
-                    check(hipEventElapsedTime(&ms, sim->events[6], sim->events[7]), "hipEventElapsedTime");
-                    ms /= static_cast<float>(trials);
-                    if (ms < best) {
-                        best = ms;
-                        best_rows = r;
-                        best_depth = d;
-                        best_flags = f;
-                    }
-                }
+            for (uint32_t d : depth_choices) {
+                if (d > 2 * sim->grid.extent[2]) continue;  // the same slab as a shorter depth
+                for (int r : row_choices) try_shape(r, d, f);
+                for (int r : pair_row_choices) try_shape(r, d, f | DP_TILE_TWO_STEP);
+            }
         }
         check(hipStreamSynchronize(sim->stream), "hipStreamSynchronize");  // before the scratch goes
         dp_sim::Tiling &tuned = sim->tiling[sim->accumulating ? 1 : 0];

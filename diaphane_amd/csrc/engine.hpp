@@ -265,10 +265,13 @@ struct dp_sim {
         return depth;
     }
 
-    void fill_tiles(StepArgs &a, int r, uint32_t owned_x, const Tiling &t) const {
+    // `halo_rows`: register rows a wave does not own (2 for one step per
+    // launch, 4 for two).
+    void fill_tiles(StepArgs &a, int r, uint32_t owned_x, const Tiling &t, int halo_rows = 2) const {
         const uint32_t nx = grid.extent[0], ny = grid.extent[1], nz = grid.extent[2];
+        const uint32_t owned_y = uint32_t(r - halo_rows);
         a.tiles_x = (nx + owned_x - 1) / owned_x;
-        a.tiles_y = (ny + (r - 2) - 1) / (r - 2);
+        a.tiles_y = (ny + owned_y - 1) / owned_y;
         a.depth = std::min<uint32_t>(nz, t.depth ? t.depth : default_depth(a.tiles_x * a.tiles_y));
         a.tiles_z = (nz + a.depth - 1) / a.depth;
         a.tiles_total = a.tiles_x * a.tiles_y * a.tiles_z;
@@ -280,6 +283,27 @@ struct dp_sim {
         }
     }
 
+    // Register rows of the two-step kernel for a tiling (6, 8 or 10).
+    static int pair_rows(const Tiling &t) { return t.rows == 6 || t.rows == 10 ? t.rows : 8; }
+
+    // One launch of the two-step kernel: steps `first` and `first + 1` of
+    // `table`.
+    void launch_pair(const std::vector<StepInjections> &table, uint64_t first, const InjDesc *device_inj) {
+        const Tiling &tl = active();
+        const int pr = pair_rows(tl);
+        StepArgs a = args(current, 1 - current);
+        a.injections = device_inj + table[first].offset;
+        a.n_magnetic = table[first].magnetic;
+        a.n_electric = table[first].electric;
+        a.injections2 = device_inj + table[first + 1].offset;
+        a.n_magnetic2 = table[first + 1].magnetic;
+        a.n_electric2 = table[first + 1].electric;
+        a.intensity = accumulating ? intensity.as<double>() : nullptr;
+        fill_tiles(a, pr, kPairOwnedX, tl, 4);
+        launch_fused_pair(a, pr, (tl.flags & DP_TILE_NONTEMPORAL) != 0, (tl.flags & DP_TILE_BUFFER) != 0, stream);
+        current = 1 - current;
+    }
+
     void advance(uint64_t steps) {
         if (steps == 0) return;
         std::vector<InjDesc> all;
@@ -288,7 +312,16 @@ struct dp_sim {
         const InjDesc *device_inj = injections.as<InjDesc>();
         if (accumulating) ensure_intensity();
         const int r = effective_rows();
+        const bool pairs = kernel == DP_KERNEL_FUSED && (active().flags & DP_TILE_TWO_STEP);
         for (uint64_t s = 0; s < steps; ++s) {
+            if (pairs && s + 1 < steps) {
+                launch_pair(table, s, device_inj);
+                check(hipGetLastError(), "kernel launch");
+                ++launches;
+                if (accumulating) accumulated += 2;
+                ++s;
+                continue;
+            }
             const auto &t = table[s];
             if (kernel == DP_KERNEL_FUSED || kernel == DP_KERNEL_FUSED_OVERLAP) {
                 StepArgs a = args(current, 1 - current);

@@ -8,10 +8,8 @@ namespace dpa {
 
 namespace {
 
-__device__ __forceinline__ float axis_value(const float *__restrict__ table, uint32_t span,
-                                            int section, uint32_t offset) {
-    return table[section * span + offset];
-}
+#include "device_common.inc"
+
 
 // H ← ((1−m)·H − g·curl)/(1+m) forward (src/cpu.rs:328-330) and its exact
 // inverse ((1+m)·H + g·curl)/(1−m) (src/cpu.rs:525-527). With m = 0 the
@@ -55,26 +53,7 @@ __device__ __forceinline__ float inject(float f, uint32_t component, uint32_t x,
     return f;
 }
 
-// Σ|E|² as src/cpu.rs:188-193 forms it: 0 + Ex², then + Ey², then + Ez².
-__device__ __forceinline__ float intensity_term(float ex, float ey, float ez) {
-    float sum = ex * ex;
-    sum = sum + ey * ey;
-    sum = sum + ez * ez;
-    return sum;
-}
 
-// Workgroups are dispatched to the 8 XCDs round-robin. Renumber them so each
-// XCD takes runs of `group` consecutive blocks while the eight XCDs
-// still sweep the grid together: within a run, a tile's neighbours along y
-// sit on the same XCD and share its L2. A bijection on [0, blocks); the tail
-// past the last whole group of 8·group blocks keeps its numbering.
-__device__ __forceinline__ uint32_t xcd_grouped(uint32_t block, uint32_t blocks, uint32_t group) {
-    constexpr uint32_t kXcds = 8;
-    const uint32_t span = kXcds * group;
-    if (block >= blocks / span * span) return block;
-    const uint32_t g = block / span, r = block % span;
-    return g * span + (r % kXcds) * group + r / kXcds;
-}
 
 // ---------------------------------------------------------------------------
 // Fused step: H, H-injections, E, E-injections and intensity in one pass.
@@ -295,6 +274,7 @@ __global__ __launch_bounds__(256) void k_fused(StepArgs p) {
     }
 }
 
+#include "fused_access.inc"
 #include "fused_aligned.inc"
 
 // ---------------------------------------------------------------------------

@@ -55,6 +55,11 @@ struct StepArgs {
     uint32_t tiles_x, tiles_y, tiles_z, depth;
     uint32_t tiles_total;
     uint32_t xcd_remap;  // 0, or the run length in blocks per XCD
+    // the second step of a two-step launch (k_fused_pair): its injections,
+    // laid out like the first step's
+    const InjDesc *injections2;
+    uint32_t n_magnetic2;
+    uint32_t n_electric2;
 };
 
 // One grid of a batch, as the batch kernel sees it (device memory).
@@ -96,6 +101,10 @@ constexpr int kAlignedX = 64;
 // component offsets in the scalar offset) when the grid allows it.
 void launch_fused_aligned(const StepArgs &args, int rows, bool nontemporal, bool buffer, hipStream_t stream);
 bool buffer_addressing_fits(uint64_t cells, uint32_t nx, uint32_t ny);
+// Two steps per launch: 64-lane waves owning 60 columns (two halo lanes on
+// each side) and rows − 4 of `rows` register rows (6, 8 or 10).
+constexpr int kPairOwnedX = kLanes - 4;
+void launch_fused_pair(const StepArgs &args, int rows, bool nontemporal, bool buffer, hipStream_t stream);
 // One launch advancing every active instance of a batch by one step.
 // `blocks` is the persistent grid size (workgroups of 4 waves); with
 // `direct` the launch instead has one wave per tile.

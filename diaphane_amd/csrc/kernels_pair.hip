@@ -1,0 +1,46 @@
+// Two-step fused kernel for gfx950 (its own translation unit, so the
+// single-step kernels and this one compile in parallel). See kernels.hpp for
+// the layout and the arithmetic contract.
+#include "kernels.hpp"
+
+namespace dpa {
+
+namespace {
+
+#include "device_common.inc"
+#include "fused_access.inc"
+#include "fused_pair.inc"
+
+}  // namespace
+
+template <template <bool> class Access>
+static void launch_fused_pair_as(const StepArgs &args, int rows, bool nontemporal, hipStream_t stream) {
+    const uint32_t blocks = (args.tiles_total + 3) / 4;
+#define DPA_ROWS(R)                                                                \
+    case R:                                                                        \
+        if (nontemporal)                                                           \
+            k_fused_pair<R, Access<true>><<<blocks, 256, 0, stream>>>(args);       \
+        else                                                                       \
+            k_fused_pair<R, Access<false>><<<blocks, 256, 0, stream>>>(args);      \
+        break;
+    switch (rows) {
+        DPA_ROWS(6)
+        DPA_ROWS(8)
+        DPA_ROWS(10)
+    default:
+        k_fused_pair<8, Access<true>><<<blocks, 256, 0, stream>>>(args);
+        break;
+    }
+#undef DPA_ROWS
+}
+
+void launch_fused_pair(const StepArgs &args, int rows, bool nontemporal, bool buffer, hipStream_t stream) {
+    if (buffer && buffer_addressing_fits(args.cells, args.nx, args.ny))
+        launch_fused_pair_as<BufferAccess>(args, rows, nontemporal, stream);
+    else if (args.cells * sizeof(float) <= 0xFFFFFFFFull)
+        launch_fused_pair_as<Pointer32>(args, rows, nontemporal, stream);
+    else
+        launch_fused_pair_as<Pointer64>(args, rows, nontemporal, stream);
+}
+
+}  // namespace dpa

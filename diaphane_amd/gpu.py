@@ -23,6 +23,7 @@ KERNEL_FUSED_OVERLAP = 2
 TILE_XCD_REMAP = 1
 TILE_NONTEMPORAL = 2
 TILE_BUFFER = 4
+TILE_TWO_STEP = 8
 
 BUFFER_ELECTRIC = 0
 BUFFER_MAGNETIC = 1
@@ -218,7 +219,8 @@ class Simulation:
         check(lib().dp_sim_set_kernel(self._handle, kernel))
 
     def set_tile(self, rows: int = 0, depth: int = 0, flags: int = -1) -> None:
-        """Fused-kernel tiling; flags: 1 = XCD-contiguous tiles, 2 = non-temporal stores."""
+        """Fused-kernel tiling; flags: 1 = XCD-contiguous tiles, 2 = non-temporal
+        stores, 4 = buffer addressing, 8 = two steps per launch."""
         check(lib().dp_sim_set_tile(self._handle, rows, depth, flags))
 
     def autotune(self, trials: int = 3) -> dict:

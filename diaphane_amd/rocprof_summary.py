@@ -64,7 +64,8 @@ def main(argv: Optional[List[str]] = None) -> int:
                     help="FETCH_SIZE factor measured on another run (used without a calibration kernel)")
     ap.add_argument("--write-scale", type=float, default=0.0)
     ap.add_argument("--cells", type=float, required=True)
-    ap.add_argument("--bytes-per-cell", type=float, required=True)
+    ap.add_argument("--bytes-per-cell", type=float, required=True, help="algorithmic bytes per cell-update")
+    ap.add_argument("--steps-per-launch", type=int, default=1, help="cell-updates per cell in one launch")
     ap.add_argument("--peak-gbs", type=float, default=8000.0)
     ap.add_argument("--command", default="")
     ap.add_argument("--out", required=True)
@@ -78,7 +79,7 @@ def main(argv: Optional[List[str]] = None) -> int:
     calls = sum(int(r["Calls"]) for r in stats)
     total_ns = sum(float(r["TotalDurationNs"]) for r in stats)
     avg_ms = total_ns / calls / 1e6
-    algorithmic = args.cells * args.bytes_per_cell
+    algorithmic = args.cells * args.bytes_per_cell * args.steps_per_launch
     achieved = algorithmic / (avg_ms / 1e3) / 1e9
 
     kernel_name = display_name(stats[0]["Name"]) if len({r["Name"] for r in stats}) == 1 else args.kernel
@@ -87,6 +88,7 @@ def main(argv: Optional[List[str]] = None) -> int:
         "launches": calls,
         "avg_ms": avg_ms,
         "algorithmic_bytes_per_launch": algorithmic,
+        "steps_per_launch": args.steps_per_launch,
         "achieved_gbs": achieved,
         "peak_gbs": args.peak_gbs,
         "frac": achieved / args.peak_gbs,
@@ -95,7 +97,9 @@ def main(argv: Optional[List[str]] = None) -> int:
              "| quantity | value |", "|---|---|",
              f"| launches profiled | {calls} |",
              f"| average duration | {avg_ms:.3f} ms |",
-             f"| algorithmic bytes / launch | {algorithmic / 1e9:.3f} GB ({args.bytes_per_cell:g} B × {args.cells:.0f} cells) |",
+             f"| algorithmic bytes / launch | {algorithmic / 1e9:.3f} GB ({args.bytes_per_cell:g} B × "
+             + (f"{args.steps_per_launch} steps × " if args.steps_per_launch != 1 else "")
+             + f"{args.cells:.0f} cells) |",
              f"| achieved (algorithmic / duration) | {achieved:.1f} GB/s = {100 * achieved / args.peak_gbs:.1f}% of {args.peak_gbs:.0f} GB/s |"]
 
     read_scale = args.read_scale or 1.0

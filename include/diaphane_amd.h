@@ -301,7 +301,7 @@ enum { DP_KERNEL_FUSED = 0, DP_KERNEL_TWO_PASS = 1, DP_KERNEL_FUSED_OVERLAP = 2 
 /* DP_KERNEL_FUSED uses 64-aligned tiles whose x halo is fetched by the edge
  * lanes; DP_KERNEL_FUSED_OVERLAP uses 62-wide tiles whose halo columns are
  * ordinary lanes. */
-enum { DP_TILE_XCD_REMAP = 1, DP_TILE_NONTEMPORAL = 2, DP_TILE_BUFFER = 4 };
+enum { DP_TILE_XCD_REMAP = 1, DP_TILE_NONTEMPORAL = 2, DP_TILE_BUFFER = 4, DP_TILE_TWO_STEP = 8 };
 int dp_sim_set_kernel(dp_sim *sim, int kernel);
 /* Fused-kernel tiling: rows per wave along y (4, 6, 8, 10 or 12), z-slab
  * depth in planes, and flags:
@@ -311,7 +311,14 @@ int dp_sim_set_kernel(dp_sim *sim, int kernel);
  *   DP_TILE_NONTEMPORAL   streams the field stores past the caches;
  *   DP_TILE_BUFFER        addresses the fields through buffer resources with
  *                         32-bit lane offsets (used when the grid's offsets
- *                         fit).
+ *                         fit);
+ *   DP_TILE_TWO_STEP      advances two steps per launch: the first step's
+ *                         fields stay in registers and the second consumes
+ *                         them one z-plane behind, so each pair of steps
+ *                         reads and writes the fields once. Waves own 60
+ *                         columns and rows − 4 of their rows (6, 8 or 10;
+ *                         other values select 8); an odd step left over in
+ *                         an advance runs single.
  * 0 for rows selects 4; 0 for depth selects a size-aware default (the power
  * of two at or below the number of x-y tiles / 50, within 1..32: short slabs
  * give a small grid enough waves, long ones amortize a large grid's halo
@@ -319,10 +326,10 @@ int dp_sim_set_kernel(dp_sim *sim, int kernel);
  * it. */
 int dp_sim_set_tile(dp_sim *sim, int rows, int depth, int flags);
 /* Autotuner for DP_KERNEL_FUSED: times `trials` launches of every rows
- * (4, 6, 8) × depth (1 to 64 in powers of two, up to twice the grid's z
- * extent) × addressing (pointer, buffer where it
- * fits) candidate on this sim's grid, keeps the fastest and reports it (any
- * out pointer may be null). A sim keeps one tiling for plain steps and one
+ * (4, 6, 8 one step per launch; 6, 8, 10 two steps per launch) × depth (1 to
+ * 64 in powers of two, up to twice the grid's z extent) × addressing
+ * (pointer, buffer where it fits) candidate on this sim's grid, keeps the
+ * fastest per step and reports it (any out pointer may be null). A sim keeps one tiling for plain steps and one
  * for intensity-accumulating steps; the autotuner sets the one for the
  * sim's current mode (dp_sim_set_tile sets both). The state is unchanged:
  * trial launches read the current fields, write only the idle half of each

@@ -14,7 +14,7 @@ import pytest
 import oracle
 from diaphane_amd.batch import Batch
 from diaphane_amd.gpu import KERNEL_FUSED, KERNEL_FUSED_OVERLAP, KERNEL_TWO_PASS, TILE_BUFFER, TILE_NONTEMPORAL, \
-    TILE_XCD_REMAP, Simulation
+    TILE_TWO_STEP, TILE_XCD_REMAP, Simulation
 
 from random_scenes import random_case
 
@@ -37,6 +37,8 @@ def run_plan(seed: int):
     flags = int(rng.choice([0, TILE_NONTEMPORAL, TILE_BUFFER, TILE_NONTEMPORAL | TILE_BUFFER]))
     if rng.random() < 0.3:
         flags |= TILE_XCD_REMAP | (int(rng.choice([0, 2, 4])) << 8)
+    if rng.random() < 0.5:
+        flags |= TILE_TWO_STEP  # two steps per launch (the fused kernel's option)
     return kernel, rows, depth, flags, bool(seed % 2)
 
 
