@@ -264,6 +264,15 @@ struct dp_sim {
         while (depth < 32 && depth * 2 * 50 <= tiles_xy) depth *= 2;
         return depth;
     }
+    // Two steps per launch re-read two halo planes per slab instead of one,
+    // so they favour slabs about twice as deep: the power of two at or below
+    // tiles_xy / 25, within 2..32 (measured best on MI355X: 16 at 384³ and
+    // 512³, 32 at 768³).
+    static uint32_t default_pair_depth(uint32_t tiles_xy) {
+        uint32_t depth = 2;
+        while (depth < 32 && depth * 2 * 25 <= tiles_xy) depth *= 2;
+        return depth;
+    }
 
     // `halo_rows`: register rows a wave does not own (2 for one step per
     // launch, 4 for two).
@@ -272,7 +281,9 @@ struct dp_sim {
         const uint32_t owned_y = uint32_t(r - halo_rows);
         a.tiles_x = (nx + owned_x - 1) / owned_x;
         a.tiles_y = (ny + owned_y - 1) / owned_y;
-        a.depth = std::min<uint32_t>(nz, t.depth ? t.depth : default_depth(a.tiles_x * a.tiles_y));
+        const uint32_t tiles_xy = a.tiles_x * a.tiles_y;
+        a.depth = std::min<uint32_t>(
+            nz, t.depth ? t.depth : (halo_rows == 4 ? default_pair_depth(tiles_xy) : default_depth(tiles_xy)));
         a.tiles_z = (nz + a.depth - 1) / a.depth;
         a.tiles_total = a.tiles_x * a.tiles_y * a.tiles_z;
         if (t.flags & DP_TILE_XCD_REMAP) {

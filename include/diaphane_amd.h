@@ -320,16 +320,18 @@ int dp_sim_set_kernel(dp_sim *sim, int kernel);
  *                         other values select 8); an odd step left over in
  *                         an advance runs single.
  * 0 for rows selects 4; 0 for depth selects a size-aware default (the power
- * of two at or below the number of x-y tiles / 50, within 1..32: short slabs
- * give a small grid enough waves, long ones amortize a large grid's halo
- * planes). flags < 0 keeps the current flags. Results never depend on any of
+ * of two at or below the number of x-y tiles / 50, within 1..32, or / 25
+ * within 2..32 for two steps per launch: short slabs give a small grid
+ * enough waves, long ones amortize a large grid's halo planes). flags < 0 keeps the current flags. Results never depend on any of
  * it. */
 int dp_sim_set_tile(dp_sim *sim, int rows, int depth, int flags);
 /* Autotuner for DP_KERNEL_FUSED: times `trials` launches of every rows
- * (4, 6, 8 one step per launch; 6, 8, 10 two steps per launch) × depth (1 to
- * 64 in powers of two, up to twice the grid's z extent) × addressing
- * (pointer, buffer where it fits) candidate on this sim's grid, keeps the
- * fastest per step and reports it (any out pointer may be null). A sim keeps one tiling for plain steps and one
+ * (4, 6, 8 one step per launch with non-temporal stores; 6, 8, 10 two steps
+ * per launch with either store kind) × depth (1 to 64 in powers of two, up
+ * to twice the grid's z extent) × addressing (pointer, buffer where it fits)
+ * candidate on this sim's grid, times the four fastest again with 3·trials
+ * launches in two alternating rounds, keeps the fastest per step and reports
+ * it (any out pointer may be null). A sim keeps one tiling for plain steps and one
  * for intensity-accumulating steps; the autotuner sets the one for the
  * sim's current mode (dp_sim_set_tile sets both). The state is unchanged:
  * trial launches read the current fields, write only the idle half of each
