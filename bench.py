@@ -424,6 +424,8 @@ def main() -> int:
     phase_a = accumulate_from if accumulate_from is not None else steps
     phase_b = steps - phase_a
 
+    host_e = np.empty(3 * cells, dtype=np.float32) if readback else None  # the application's readback buffer
+
     def run(count: int) -> None:
         if not readback:
             sim.advance_by(count)
@@ -432,7 +434,7 @@ def main() -> int:
         while done < count:
             chunk = min(readback, count - done)
             sim.advance_by(chunk)
-            sim.read_electric()
+            sim.read_electric(out=host_e)
             done += chunk
 
     launches = {"plain": 0, "with_intensity": 0}

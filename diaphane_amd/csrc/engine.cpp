@@ -193,6 +193,14 @@ int dp_sim_create(dp_engine *engine, const dp_scene *scene, dp_sim **out) {
                 fail(DP_ERR_INVALID, "source has an unknown shape or axis");
         }
         sim->cells = g.cells();
+        // Large grids default to two steps per launch (measured on MI355X:
+        // faster from about 384³ up, slower at 256³), where buffer
+        // addressing fits; smaller grids keep one step per launch.
+        if (sim->cells >= dp_sim::kTwoStepDefaultCells &&
+            buffer_addressing_fits(sim->cells, g.extent[0], g.extent[1])) {
+            sim->tiling[0] = {10, 0, DP_TILE_TWO_STEP | DP_TILE_BUFFER};
+            sim->tiling[1] = {6, 0, DP_TILE_TWO_STEP | DP_TILE_BUFFER};
+        }
         sim->time_step = g.time_step();
         sim->material_count = scene->material_count;
         sim->sources.assign(scene->sources, scene->sources + scene->source_count);

@@ -142,6 +142,7 @@ struct dp_sim {
         int flags = DP_TILE_NONTEMPORAL;
     };
     Tiling tiling[2];
+    static constexpr uint64_t kTwoStepDefaultCells = 32ull << 20;
     const Tiling &active() const { return tiling[accumulating ? 1 : 0]; }
     uint64_t launches = 0;
     hipEvent_t events[8] = {};

@@ -166,13 +166,22 @@ class Simulation:
     def accumulated_steps(self) -> int:
         return int(lib().dp_sim_accumulated_steps(self._handle))
 
-    def read_electric(self) -> np.ndarray:
-        out = np.empty(3 * self.cells, dtype=np.float32)
+    def _field_out(self, out: Optional[np.ndarray]) -> np.ndarray:
+        if out is None:
+            return np.empty(3 * self.cells, dtype=np.float32)
+        if out.dtype != np.float32 or out.size != 3 * self.cells or not out.flags.c_contiguous:
+            raise DiaphaneError(-4, "out must be a contiguous float32 array of 3 x cell count")
+        return out
+
+    def read_electric(self, out: Optional[np.ndarray] = None) -> np.ndarray:
+        """E, component-major; into `out` when given (a reused host buffer
+        skips the page faults of a fresh one)."""
+        out = self._field_out(out)
         check(lib().dp_sim_read_electric(self._handle, _fptr(out), out.size))
         return out
 
-    def read_magnetic(self) -> np.ndarray:
-        out = np.empty(3 * self.cells, dtype=np.float32)
+    def read_magnetic(self, out: Optional[np.ndarray] = None) -> np.ndarray:
+        out = self._field_out(out)
         check(lib().dp_sim_read_magnetic(self._handle, _fptr(out), out.size))
         return out
 

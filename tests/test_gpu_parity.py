@@ -300,3 +300,24 @@ def test_restore_then_advance_matches(context):
     reference.advance_by(50)
     candidate.advance_by(50)
     assert_parity(reference, candidate, 150)
+
+
+def test_readback_into_a_reused_host_buffer(context):
+    import numpy as np
+    from diaphane_amd import Extent, Scene
+    from diaphane_amd._lib import DiaphaneError
+    from diaphane_amd.gpu import Simulation
+
+    sim = Simulation(context, Scene.photon(Extent(40, 32, 30)))
+    buffer = np.full(3 * sim.cells, np.nan, dtype=np.float32)
+    for _ in range(3):
+        sim.advance_by(5)
+        fresh = sim.read_electric()
+        assert sim.read_electric(out=buffer) is buffer
+        assert np.array_equal(fresh.view(np.uint32), buffer.view(np.uint32))
+    assert np.array_equal(sim.read_magnetic(out=buffer).view(np.uint32), sim.read_magnetic().view(np.uint32))
+    for wrong in (np.empty(3 * sim.cells, np.float64), np.empty(sim.cells, np.float32),
+                  np.empty((3 * sim.cells, 2), np.float32)[:, 0]):
+        with pytest.raises(DiaphaneError):
+            sim.read_electric(out=wrong)
+    sim.close()
