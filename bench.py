@@ -222,9 +222,11 @@ def bench_batched(args) -> int:
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
             "traffic": None,
-            "kernel": "k_fused_batch",
+            "kernel": "k_fused_pair_batch" if tuned.get("two_step") else
+                      ("k_fused_batch_direct" if tuned.get("wave_per_tile") else "k_fused_batch"),
             "bytes_per_cell": BYTES_PER_CELL,
-            "avg_launch_ms": batch_ms / steps,
+            "launches": batch.launch_count(),
+            "avg_launch_ms": batch_ms / max(batch.launch_count(), 1),
         },
     }
     batch.close()

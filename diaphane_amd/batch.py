@@ -67,8 +67,9 @@ class Batch:
         check(lib().dp_batch_set_tile(self._handle, rows, depth, flags))
 
     def autotune(self, trials: int = 5) -> tuple[int, int, float]:
-        """Times every rows × depth candidate; keeps and returns the fastest
-        as (rows, depth, ms per step). Member state is unchanged."""
+        """Times every rows × depth × form candidate (one or two steps per
+        launch); keeps and returns the fastest as (rows, depth, ms per step).
+        Member state is unchanged."""
         rows, depth, ms = c_int(), c_int(), c_float()
         check(lib().dp_batch_autotune(self._handle, trials, byref(rows), byref(depth), byref(ms)))
         return rows.value, depth.value, ms.value
@@ -77,7 +78,8 @@ class Batch:
         rows, depth, flags = c_int(), c_int(), c_int()
         check(lib().dp_batch_tiling(self._handle, byref(rows), byref(depth), byref(flags)))
         return {"rows": rows.value, "depth": depth.value, "wave_per_tile": bool(flags.value & WAVE_PER_TILE),
-                "buffer": bool(flags.value & 4), "nontemporal": bool(flags.value & 2)}
+                "buffer": bool(flags.value & 4), "nontemporal": bool(flags.value & 2),
+                "two_step": bool(flags.value & 8)}
 
     def __len__(self) -> int:
         return int(lib().dp_batch_size(self._handle))

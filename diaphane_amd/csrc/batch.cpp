@@ -99,7 +99,11 @@ struct dp_batch {
         dirty = true;
     }
 
-    bool direct() const { return (flags & DP_BATCH_WAVE_PER_TILE) != 0; }
+    // Two steps per launch use one wave per tile.
+    bool two_step() const { return (flags & DP_TILE_TWO_STEP) != 0; }
+    bool direct() const { return (flags & DP_BATCH_WAVE_PER_TILE) != 0 || two_step(); }
+    // Launches that advance a member by `steps` steps.
+    uint32_t launches_for(uint32_t steps) const { return two_step() ? (steps + 1) / 2 : steps; }
     // Buffer addressing when asked for and every member's offsets fit it.
     bool buffer() const {
         if (!(flags & DP_TILE_BUFFER)) return false;
@@ -116,10 +120,13 @@ struct dp_batch {
 
     void sync() { check(hipStreamSynchronize(stream), "hipStreamSynchronize"); }
 
+    uint32_t owned_x() const { return two_step() ? kPairOwnedX : kAlignedX; }
+    int owned_rows() const { return rows - (two_step() ? 4 : 2); }
+
     void tiles_of(const dp_sim *sim, InstanceDesc &d) const {
         const uint32_t nx = sim->grid.extent[0], ny = sim->grid.extent[1], nz = sim->grid.extent[2];
-        d.tiles_x = (nx + kAlignedX - 1) / kAlignedX;
-        d.tiles_y = (ny + (rows - 2) - 1) / (rows - 2);
+        d.tiles_x = (nx + owned_x() - 1) / owned_x();
+        d.tiles_y = (ny + owned_rows() - 1) / owned_rows();
         d.depth = std::min<uint32_t>(nz, slab);
         d.tiles_z = (nz + d.depth - 1) / d.depth;
         d.tiles_total = d.tiles_x * d.tiles_y * d.tiles_z;
@@ -131,10 +138,10 @@ struct dp_batch {
     uint32_t auto_depth() const {
         uint64_t tiles_xy = 0;
         for (const dp_sim *s : members)
-            tiles_xy += uint64_t((s->grid.extent[0] + kAlignedX - 1) / kAlignedX) *
-                        ((s->grid.extent[1] + (rows - 2) - 1) / (rows - 2));
+            tiles_xy += uint64_t((s->grid.extent[0] + owned_x() - 1) / owned_x()) *
+                        ((s->grid.extent[1] + owned_rows() - 1) / owned_rows());
         const uint32_t capped = static_cast<uint32_t>(std::min<uint64_t>(tiles_xy, 0xFFFFFFFFull));
-        return std::min<uint32_t>(16, dp_sim::default_depth(capped));
+        return std::min<uint32_t>(16, two_step() ? dp_sim::default_pair_depth(capped) : dp_sim::default_depth(capped));
     }
 
     // Instance table and tile owners for the current membership.
@@ -201,8 +208,14 @@ struct dp_batch {
         if (!trial_injections.ptr) trial_injections.allocate(sizeof(InjDesc));
         const bool nt = (flags & DP_TILE_NONTEMPORAL) != 0;
         auto launch = [&] {
-            launch_fused_batch(instance_table.as<InstanceDesc>(), trial_slots.as<StepSlot>(), owners.as<uint32_t>(),
-                               n_tiles, trial_injections.as<InjDesc>(), rows, nt, buffer(), direct(), blocks, stream);
+            if (two_step())
+                launch_fused_pair_batch(instance_table.as<InstanceDesc>(), trial_slots.as<StepSlot>(),
+                                        owners.as<uint32_t>(), n_tiles, trial_injections.as<InjDesc>(), rows, nt,
+                                        buffer(), stream);
+            else
+                launch_fused_batch(instance_table.as<InstanceDesc>(), trial_slots.as<StepSlot>(), owners.as<uint32_t>(),
+                                   n_tiles, trial_injections.as<InjDesc>(), rows, nt, buffer(), direct(), blocks,
+                                   stream);
         };
         launch();  // first launch of a shape loads its code object
         check(hipGetLastError(), "kernel launch");
@@ -212,24 +225,27 @@ struct dp_batch {
         check(hipEventSynchronize(events[7]), "hipEventSynchronize");
         float ms = 0.0f;
         check(hipEventElapsedTime(&ms, events[6], events[7]), "hipEventElapsedTime");
-        return ms / static_cast<float>(trials);
+        return ms / static_cast<float>(two_step() ? 2 * trials : trials);  // per step
     }
 
     // Picks the grid form, rows and slab depth by timing every candidate on
     // the members as they are. Results never depend on the choice.
     void autotune(uint32_t trials, int *rows_out, int *depth_out, float *ms_out) {
         static const int row_choices[] = {4, 6, 8};
+        static const int pair_row_choices[] = {6, 8, 10};
         static const uint32_t depth_choices[] = {8, 16, 32, 64};
         static const int grid_choices[] = {0, DP_BATCH_WAVE_PER_TILE, DP_TILE_BUFFER,
-                                           DP_BATCH_WAVE_PER_TILE | DP_TILE_BUFFER};
-        const int tuned_bits = DP_BATCH_WAVE_PER_TILE | DP_TILE_BUFFER;
+                                           DP_BATCH_WAVE_PER_TILE | DP_TILE_BUFFER,
+                                           DP_TILE_TWO_STEP, DP_TILE_TWO_STEP | DP_TILE_BUFFER};
+        const int tuned_bits = DP_BATCH_WAVE_PER_TILE | DP_TILE_BUFFER | DP_TILE_TWO_STEP;
         int best_rows = rows, best_grid = flags & tuned_bits;
         uint32_t best_depth = depth;
         float best = std::numeric_limits<float>::infinity();
         if (!members.empty()) {
             for (int g : grid_choices)
-                for (int r : row_choices)
+                for (int k = 0; k < 3; ++k)
                     for (uint32_t d : depth_choices) {
+                        const int r = (g & DP_TILE_TWO_STEP) ? pair_row_choices[k] : row_choices[k];
                         flags = (flags & ~tuned_bits) | g;
                         rows = r;
                         depth = d;
@@ -253,16 +269,24 @@ struct dp_batch {
         if (ms_out) *ms_out = members.empty() ? 0.0f : best;
     }
 
-    void launch_chunk(StagingSet &set, uint32_t steps) {
+    // `steps` launches (one or two steps each, by the tiling); a full chunk
+    // (`capture`) runs as a captured graph.
+    void launch_chunk(StagingSet &set, uint32_t steps, bool capture) {
         const StepSlot *slots = set.slots.as<StepSlot>();
         const InjDesc *inj = set.injections.as<InjDesc>();
         const bool nt = (flags & DP_TILE_NONTEMPORAL) != 0;
         auto launch_all = [&] {
-            for (uint32_t s = 0; s < steps; ++s)
-                launch_fused_batch(instance_table.as<InstanceDesc>(), slots + size_t(s) * members.size(),
-                                   owners.as<uint32_t>(), n_tiles, inj, rows, nt, buffer(), direct(), blocks, stream);
+            for (uint32_t s = 0; s < steps; ++s) {
+                if (two_step())
+                    launch_fused_pair_batch(instance_table.as<InstanceDesc>(), slots + size_t(s) * members.size(),
+                                            owners.as<uint32_t>(), n_tiles, inj, rows, nt, buffer(), stream);
+                else
+                    launch_fused_batch(instance_table.as<InstanceDesc>(), slots + size_t(s) * members.size(),
+                                       owners.as<uint32_t>(), n_tiles, inj, rows, nt, buffer(), direct(), blocks,
+                                       stream);
+            }
         };
-        if (graph_steps == 0 || steps != graph_steps) {
+        if (!capture) {
             launch_all();
             check(hipGetLastError(), "kernel launch");
             return;
@@ -303,7 +327,8 @@ struct dp_batch {
             if (!any) return;
             const uint32_t chunk = static_cast<uint32_t>(
                 std::min<uint64_t>(steps, graph_steps ? graph_steps : 64));
-            slot_buf.assign(size_t(chunk) * n, StepSlot{});
+            const uint32_t chunk_launches = launches_for(chunk);
+            slot_buf.assign(size_t(chunk_launches) * n, StepSlot{});
             inj_all.clear();
             for (size_t i = 0; i < n; ++i) {
                 dp_sim *sim = members[i];
@@ -311,15 +336,30 @@ struct dp_batch {
                 const uint32_t active = static_cast<uint32_t>(std::min<uint64_t>(room, chunk));
                 if (active == 0) continue;
                 auto table = sim->build_injections(sim->step, active, +1, inj_one);
+                const uint32_t base = static_cast<uint32_t>(inj_all.size());
                 int parity = sim->current;
-                for (uint32_t s = 0; s < active; ++s) {
+                // Launch s advances the member by steps s (one step per
+                // launch), or by steps 2s and 2s + 1 (two), the last of an
+                // odd count alone.
+                const uint32_t per_launch = two_step() ? 2 : 1;
+                for (uint32_t s = 0; s < launches_for(active); ++s) {
+                    const uint32_t first = s * per_launch;
                     StepSlot &slot = slot_buf[size_t(s) * n + i];
                     slot.active = 1;
                     slot.parity = static_cast<uint32_t>(parity);
-                    slot.inj_offset = static_cast<uint32_t>(inj_all.size()) + table[s].offset;
-                    slot.n_magnetic = table[s].magnetic;
-                    slot.n_electric = table[s].electric;
+                    slot.inj_offset = base + table[first].offset;
+                    slot.n_magnetic = table[first].magnetic;
+                    slot.n_electric = table[first].electric;
                     slot.accumulate = sim->accumulating ? 1u : 0u;
+                    if (two_step()) {
+                        if (first + 1 < active) {
+                            slot.inj_offset2 = base + table[first + 1].offset;
+                            slot.n_magnetic2 = table[first + 1].magnetic;
+                            slot.n_electric2 = table[first + 1].electric;
+                        } else {
+                            slot.single = 1;
+                        }
+                    }
                     parity = 1 - parity;
                 }
                 inj_all.insert(inj_all.end(), inj_one.begin(), inj_one.end());
@@ -350,18 +390,19 @@ struct dp_batch {
             check(hipMemcpyAsync(set.injections.ptr, static_cast<char *>(set.host.ptr) + slot_bytes, inj_bytes,
                                  hipMemcpyHostToDevice, stream),
                   "hipMemcpyAsync");
-            launch_chunk(set, chunk);
+            launch_chunk(set, chunk_launches, graph_steps != 0 && chunk == graph_steps);
             check(hipEventRecord(set.done, stream), "hipEventRecord");
             set.in_flight = true;
-            launches += chunk;
+            launches += chunk_launches;
             for (size_t i = 0; i < n; ++i) {
                 dp_sim *sim = members[i];
                 const uint64_t room = targets[i] > sim->step ? targets[i] - sim->step : 0;
                 const uint32_t active = static_cast<uint32_t>(std::min<uint64_t>(room, chunk));
+                const uint32_t member_launches = launches_for(active);
                 sim->step += active;
-                if (active & 1) sim->current = 1 - sim->current;
+                if (member_launches & 1) sim->current = 1 - sim->current;  // each launch flips the pair once
                 if (sim->accumulating) sim->accumulated += active;
-                sim->launches += active;
+                sim->launches += member_launches;
             }
             steps -= chunk;
         }
@@ -466,12 +507,17 @@ int dp_batch_autotune(dp_batch *batch, uint32_t trials, int *rows, int *depth, f
 int dp_batch_set_tile(dp_batch *batch, int rows, int depth, int flags) {
     return guarded([&] {
         if (!batch) fail(DP_ERR_INVALID, "null batch");
-        if (rows != 0 && rows != 4 && rows != 6 && rows != 8) fail(DP_ERR_INVALID, "rows must be 4, 6 or 8");
         if (depth < 0) fail(DP_ERR_INVALID, "depth must be non-negative");
+        const int new_flags =
+            flags >= 0 ? flags & (DP_TILE_NONTEMPORAL | DP_TILE_BUFFER | DP_BATCH_WAVE_PER_TILE | DP_TILE_TWO_STEP)
+                       : batch->flags;
+        const bool two = (new_flags & DP_TILE_TWO_STEP) != 0;
+        if (rows != 0 && (two ? rows != 6 && rows != 8 && rows != 10 : rows != 4 && rows != 6 && rows != 8))
+            fail(DP_ERR_INVALID, two ? "rows must be 6, 8 or 10 for two steps per launch" : "rows must be 4, 6 or 8");
         batch->sync();
-        batch->rows = rows ? rows : 4;
+        batch->flags = new_flags;
+        batch->rows = rows ? rows : (two ? 8 : 4);
         batch->depth = static_cast<uint32_t>(depth);
-        if (flags >= 0) batch->flags = flags & (DP_TILE_NONTEMPORAL | DP_TILE_BUFFER | DP_BATCH_WAVE_PER_TILE);
         batch->dirty = true;
     });
 }

@@ -60,6 +60,7 @@ struct StepArgs {
     const InjDesc *injections2;
     uint32_t n_magnetic2;
     uint32_t n_electric2;
+    uint32_t single_step;  // a two-step launch that advances this grid one step
 };
 
 // One grid of a batch, as the batch kernel sees it (device memory).
@@ -85,6 +86,12 @@ struct alignas(16) StepSlot {
     uint32_t n_magnetic;
     uint32_t n_electric;
     uint32_t accumulate;  // add |E|² into the instance's intensity
+    // two-step launches: the second step's injections, and 1 when this
+    // instance takes only one step (its last, in an odd count)
+    uint32_t inj_offset2;
+    uint32_t n_magnetic2;
+    uint32_t n_electric2;
+    uint32_t single;
     uint32_t pad[2];
 };
 
@@ -105,6 +112,11 @@ bool buffer_addressing_fits(uint64_t cells, uint32_t nx, uint32_t ny);
 // each side) and rows − 4 of `rows` register rows (6, 8 or 10).
 constexpr int kPairOwnedX = kLanes - 4;
 void launch_fused_pair(const StepArgs &args, int rows, bool nontemporal, bool buffer, hipStream_t stream);
+// Many grids, two steps per launch: one wave per tile of the packed table
+// (tiles cut for the two-step kernel).
+void launch_fused_pair_batch(const InstanceDesc *instances, const StepSlot *slots, const uint32_t *owners,
+                             uint32_t n_tiles, const InjDesc *injections, int rows, bool nontemporal, bool buffer,
+                             hipStream_t stream);
 // One launch advancing every active instance of a batch by one step.
 // `blocks` is the persistent grid size (workgroups of 4 waves); with
 // `direct` the launch instead has one wave per tile.

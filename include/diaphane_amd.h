@@ -422,22 +422,25 @@ int dp_batch_wait(dp_batch *batch, uint32_t timeout_ms);
 /* Launches per captured HIP graph (default 32; 0 launches each kernel
  * directly). Results never depend on it. */
 int dp_batch_set_graph_steps(dp_batch *batch, uint32_t steps);
-/* Rows per wave (4, 6 or 8; 0 = default 4), z-slab depth (0, the default:
- * sized to the membership, the single-sim rule over all members' x-y tiles
- * together, at most 16) and
+/* Rows per wave (4, 6 or 8; with DP_TILE_TWO_STEP 6, 8 or 10; 0 = default
+ * 4, or 8), z-slab depth (0, the default: sized to the membership, the
+ * single-sim rule over all members' x-y tiles together, at most 16) and
  * flags (< 0 keeps the current ones): DP_TILE_NONTEMPORAL, DP_TILE_BUFFER
- * (used when every member's offsets fit), and DP_BATCH_WAVE_PER_TILE, which
+ * (used when every member's offsets fit), DP_BATCH_WAVE_PER_TILE, which
  * launches one wave per tile instead of the persistent grid whose waves
- * stride over the tile list. The default is DP_TILE_NONTEMPORAL |
+ * stride over the tile list, and DP_TILE_TWO_STEP: each launch advances
+ * every member two steps (one wave per tile; a member's last step of an odd
+ * count runs alone in its launch). The default is DP_TILE_NONTEMPORAL |
  * DP_BATCH_WAVE_PER_TILE. */
 enum { DP_BATCH_WAVE_PER_TILE = 16 };
 int dp_batch_set_tile(dp_batch *batch, int rows, int depth, int flags);
 /* Autotuner: times `trials` launches of every grid form × addressing ×
- * rows × depth candidate (persistent or one wave per tile; pointer or
- * buffer; rows 4, 6, 8; depth 8, 16, 32, 64) on the members as they are, keeps the fastest and reports rows,
- * depth and time (any out pointer may be null). Member state is unchanged:
- * trial launches write only the idle half of each ping-pong pair.
- * Synchronous. */
+ * rows × depth candidate (persistent, one wave per tile, or two steps per
+ * launch; pointer or buffer; rows 4, 6, 8, or 6, 8, 10 for two steps; depth
+ * 8, 16, 32, 64) on the members as they are, keeps the fastest per step and
+ * reports rows, depth and time (any out pointer may be null; dp_batch_tiling
+ * reports the form). Member state is unchanged: trial launches write only
+ * the idle half of each ping-pong pair. Synchronous. */
 int dp_batch_autotune(dp_batch *batch, uint32_t trials, int *rows, int *depth, float *ms_per_step);
 /* The tiling in use (any out pointer may be null). */
 int dp_batch_tiling(const dp_batch *batch, int *rows, int *depth, int *flags);
