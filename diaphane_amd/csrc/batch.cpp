@@ -46,14 +46,15 @@ struct dp_batch {
     uint64_t version = 0;
 
     // Defaults measured on MI355X over 256 uneven grids (48–160 cells a
-    // side): one wave per tile beats the persistent grid by 13%, and 4 rows
-    // with 16-plane slabs is the fastest shape in that form. depth 0 sizes
-    // the slabs to the membership (auto_depth): a batch of one small grid
-    // needs short slabs to put enough waves on the device.
-    int rows = 4;
+    // side): two steps per launch with 6 rows and buffer addressing is the
+    // autotuner's pick, 1.5x the best one-step form (one wave per tile, 4
+    // rows, 16-plane slabs). depth 0 sizes the slabs to the membership
+    // (auto_depth): a batch of one small grid needs short slabs to put
+    // enough waves on the device.
+    int rows = 6;
     uint32_t depth = 0;
     uint32_t slab = 16;  // the slab depth the current tables use
-    int flags = DP_TILE_NONTEMPORAL | DP_BATCH_WAVE_PER_TILE;
+    int flags = DP_TILE_NONTEMPORAL | DP_BATCH_WAVE_PER_TILE | DP_TILE_TWO_STEP | DP_TILE_BUFFER;
     uint32_t graph_steps = 32;
 
     DeviceBuffer instance_table;
@@ -133,15 +134,16 @@ struct dp_batch {
     }
 
     // The slab depth for depth 0: the single-sim rule applied to the x-y
-    // tiles of every member together, at most 16 (the measured best for a
-    // full batch).
+    // tiles of every member together, at most 16 for one step per launch and
+    // 32 for two (the measured best for a full batch).
     uint32_t auto_depth() const {
         uint64_t tiles_xy = 0;
         for (const dp_sim *s : members)
             tiles_xy += uint64_t((s->grid.extent[0] + owned_x() - 1) / owned_x()) *
                         ((s->grid.extent[1] + owned_rows() - 1) / owned_rows());
         const uint32_t capped = static_cast<uint32_t>(std::min<uint64_t>(tiles_xy, 0xFFFFFFFFull));
-        return std::min<uint32_t>(16, two_step() ? dp_sim::default_pair_depth(capped) : dp_sim::default_depth(capped));
+        return two_step() ? std::min<uint32_t>(32, dp_sim::default_pair_depth(capped))
+                          : std::min<uint32_t>(16, dp_sim::default_depth(capped));
     }
 
     // Instance table and tile owners for the current membership.
@@ -516,7 +518,7 @@ int dp_batch_set_tile(dp_batch *batch, int rows, int depth, int flags) {
             fail(DP_ERR_INVALID, two ? "rows must be 6, 8 or 10 for two steps per launch" : "rows must be 4, 6 or 8");
         batch->sync();
         batch->flags = new_flags;
-        batch->rows = rows ? rows : (two ? 8 : 4);
+        batch->rows = rows ? rows : (two ? 6 : 4);
         batch->depth = static_cast<uint32_t>(depth);
         batch->dirty = true;
     });

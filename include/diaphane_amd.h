@@ -423,15 +423,16 @@ int dp_batch_wait(dp_batch *batch, uint32_t timeout_ms);
  * directly). Results never depend on it. */
 int dp_batch_set_graph_steps(dp_batch *batch, uint32_t steps);
 /* Rows per wave (4, 6 or 8; with DP_TILE_TWO_STEP 6, 8 or 10; 0 = default
- * 4, or 8), z-slab depth (0, the default: sized to the membership, the
- * single-sim rule over all members' x-y tiles together, at most 16) and
+ * 4, or 6), z-slab depth (0, the default: sized to the membership, the
+ * single-sim rule over all members' x-y tiles together, at most 16, or 32
+ * for two steps) and
  * flags (< 0 keeps the current ones): DP_TILE_NONTEMPORAL, DP_TILE_BUFFER
  * (used when every member's offsets fit), DP_BATCH_WAVE_PER_TILE, which
  * launches one wave per tile instead of the persistent grid whose waves
  * stride over the tile list, and DP_TILE_TWO_STEP: each launch advances
  * every member two steps (one wave per tile; a member's last step of an odd
- * count runs alone in its launch). The default is DP_TILE_NONTEMPORAL |
- * DP_BATCH_WAVE_PER_TILE. */
+ * count runs alone in its launch). The default is two steps per launch, 6
+ * rows, DP_TILE_NONTEMPORAL | DP_TILE_BUFFER. */
 enum { DP_BATCH_WAVE_PER_TILE = 16 };
 int dp_batch_set_tile(dp_batch *batch, int rows, int depth, int flags);
 /* Autotuner: times `trials` launches of every grid form × addressing ×
