@@ -339,7 +339,7 @@ def bench_served(args) -> int:
             "kinds": kinds,
             "hbm_budget_bytes": 200 << 30,
             "chunk_steps": args.chunk_steps,
-            "chunk_cells": "every member" if args.chunk_cells < 0 else (args.chunk_cells or 32 << 20),
+            "chunk_cells": "every member" if args.chunk_cells < 0 else (args.chunk_cells or 64 << 20),
         },
         "served": batched,
         "sequential": sequential,

@@ -46,15 +46,15 @@ struct dp_batch {
     uint64_t version = 0;
 
     // Defaults measured on MI355X over 256 uneven grids (48–160 cells a
-    // side): two steps per launch with 6 rows and buffer addressing is the
-    // autotuner's pick, 1.5x the best one-step form (one wave per tile, 4
-    // rows, 16-plane slabs). depth 0 sizes the slabs to the membership
+    // side): two steps per launch with 6 rows, buffer addressing and cached
+    // stores is the autotuner's pick, 1.6x the best one-step form (one wave
+    // per tile, 4 rows, 16-plane slabs). depth 0 sizes the slabs to the membership
     // (auto_depth): a batch of one small grid needs short slabs to put
     // enough waves on the device.
     int rows = 6;
     uint32_t depth = 0;
     uint32_t slab = 16;  // the slab depth the current tables use
-    int flags = DP_TILE_NONTEMPORAL | DP_BATCH_WAVE_PER_TILE | DP_TILE_TWO_STEP | DP_TILE_BUFFER;
+    int flags = DP_BATCH_WAVE_PER_TILE | DP_TILE_TWO_STEP | DP_TILE_BUFFER;
     uint32_t graph_steps = 32;
 
     DeviceBuffer instance_table;
@@ -236,10 +236,18 @@ struct dp_batch {
         static const int row_choices[] = {4, 6, 8};
         static const int pair_row_choices[] = {6, 8, 10};
         static const uint32_t depth_choices[] = {8, 16, 32, 64};
-        static const int grid_choices[] = {0, DP_BATCH_WAVE_PER_TILE, DP_TILE_BUFFER,
-                                           DP_BATCH_WAVE_PER_TILE | DP_TILE_BUFFER,
-                                           DP_TILE_TWO_STEP, DP_TILE_TWO_STEP | DP_TILE_BUFFER};
-        const int tuned_bits = DP_BATCH_WAVE_PER_TILE | DP_TILE_BUFFER | DP_TILE_TWO_STEP;
+        // One step per launch keeps non-temporal stores (measured faster);
+        // two steps per launch try both store kinds.
+        static const int grid_choices[] = {
+            DP_TILE_NONTEMPORAL,
+            DP_TILE_NONTEMPORAL | DP_BATCH_WAVE_PER_TILE,
+            DP_TILE_NONTEMPORAL | DP_TILE_BUFFER,
+            DP_TILE_NONTEMPORAL | DP_BATCH_WAVE_PER_TILE | DP_TILE_BUFFER,
+            DP_TILE_TWO_STEP,
+            DP_TILE_TWO_STEP | DP_TILE_BUFFER,
+            DP_TILE_TWO_STEP | DP_TILE_NONTEMPORAL,
+            DP_TILE_TWO_STEP | DP_TILE_NONTEMPORAL | DP_TILE_BUFFER};
+        const int tuned_bits = DP_BATCH_WAVE_PER_TILE | DP_TILE_BUFFER | DP_TILE_TWO_STEP | DP_TILE_NONTEMPORAL;
         int best_rows = rows, best_grid = flags & tuned_bits;
         uint32_t best_depth = depth;
         float best = std::numeric_limits<float>::infinity();

@@ -286,7 +286,10 @@ int dp_server_create(dp_engine *engine, const dp_server_config *config, dp_serve
         if (c.chunk_steps == 0) c.chunk_steps = 32;
         if (c.host_threads == 0) c.host_threads = 4;
         if (c.host_threads > 64) fail(DP_ERR_INVALID, "at most 64 host threads");
-        if (c.chunk_cells == 0) c.chunk_cells = 32ull << 20;
+        // 64M cells a chunk: with two-step batches, 1.07x the throughput of
+        // 32M and a lower p99 (4.5 vs 6.1 s) on the 50 req/s stream, at a
+        // higher p50 (2.3 vs 1.4 s), measured on MI355X.
+        if (c.chunk_cells == 0) c.chunk_cells = 64ull << 20;
         server->config = c;
         if (!c.one_at_a_time) {
             if (dp_batch_create(engine, &server->batch) != DP_OK) throw Error{DP_ERR_DEVICE, last_error()};

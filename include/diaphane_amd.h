@@ -432,13 +432,13 @@ int dp_batch_set_graph_steps(dp_batch *batch, uint32_t steps);
  * stride over the tile list, and DP_TILE_TWO_STEP: each launch advances
  * every member two steps (one wave per tile; a member's last step of an odd
  * count runs alone in its launch). The default is two steps per launch, 6
- * rows, DP_TILE_NONTEMPORAL | DP_TILE_BUFFER. */
+ * rows, DP_TILE_BUFFER, cached stores. */
 enum { DP_BATCH_WAVE_PER_TILE = 16 };
 int dp_batch_set_tile(dp_batch *batch, int rows, int depth, int flags);
 /* Autotuner: times `trials` launches of every grid form × addressing ×
  * rows × depth candidate (persistent, one wave per tile, or two steps per
- * launch; pointer or buffer; rows 4, 6, 8, or 6, 8, 10 for two steps; depth
- * 8, 16, 32, 64) on the members as they are, keeps the fastest per step and
+ * launch with either store kind; pointer or buffer; rows 4, 6, 8, or 6, 8,
+ * 10 for two steps; depth 8, 16, 32, 64) on the members as they are, keeps the fastest per step and
  * reports rows, depth and time (any out pointer may be null; dp_batch_tiling
  * reports the form). Member state is unchanged: trial launches write only
  * the idle half of each ping-pong pair. Synchronous. */
@@ -477,7 +477,7 @@ typedef struct dp_server_config {
                                   left) until this many cells step in a
                                   chunk; the others wait that chunk, and
                                   none waits more than 64 chunks in a row.
-                                  0: 32M cells; UINT64_MAX: every member
+                                  0: 64M cells; UINT64_MAX: every member
                                   steps every chunk                       */
 } dp_server_config;
 enum { DP_REQUEST_QUEUED = 0, DP_REQUEST_PREPARING = 1, DP_REQUEST_READY = 2, DP_REQUEST_RUNNING = 3,

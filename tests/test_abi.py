@@ -53,3 +53,30 @@ def test_engine_creation_without_a_device_fails_loudly():
 
     with pytest.raises(_lib.DiaphaneError):
         Context(0)
+
+
+def header_enums():
+    values = {}
+    for header in (ROOT / "include").glob("*.h"):
+        text = re.sub(r"/\*.*?\*/", "", header.read_text(), flags=re.S)
+        for body in re.findall(r"enum\s*\{([^}]*)\}", text):
+            for name, value in re.findall(r"\b(DP_[A-Z0-9_]+)\s*=\s*(-?\d+)", body):
+                values[name] = int(value)
+    return values
+
+
+def test_python_constants_match_the_header():
+    from diaphane_amd import batch, gpu
+
+    enums = header_enums()
+    pairs = {
+        "DP_KERNEL_FUSED": gpu.KERNEL_FUSED, "DP_KERNEL_TWO_PASS": gpu.KERNEL_TWO_PASS,
+        "DP_KERNEL_FUSED_OVERLAP": gpu.KERNEL_FUSED_OVERLAP,
+        "DP_TILE_XCD_REMAP": gpu.TILE_XCD_REMAP, "DP_TILE_NONTEMPORAL": gpu.TILE_NONTEMPORAL,
+        "DP_TILE_BUFFER": gpu.TILE_BUFFER, "DP_TILE_TWO_STEP": gpu.TILE_TWO_STEP,
+        "DP_BUFFER_ELECTRIC": gpu.BUFFER_ELECTRIC, "DP_BUFFER_MAGNETIC": gpu.BUFFER_MAGNETIC,
+        "DP_BUFFER_INTENSITY": gpu.BUFFER_INTENSITY, "DP_BUFFER_MATERIAL_INDEX": gpu.BUFFER_MATERIAL_INDEX,
+        "DP_BATCH_WAVE_PER_TILE": batch.WAVE_PER_TILE,
+    }
+    for name, value in pairs.items():
+        assert enums.get(name) == value, f"{name}: header {enums.get(name)}, binding {value}"
