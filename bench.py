@@ -426,7 +426,14 @@ def main() -> int:
     phase_a = accumulate_from if accumulate_from is not None else steps
     phase_b = steps - phase_a
 
-    host_e = np.empty(3 * cells, dtype=np.float32) if readback else None  # the application's readback buffer
+    # The application's readback buffer: page-locked host memory from the
+    # engine, reused for every readback.
+    host_e, pinned = None, None
+    if readback:
+        from diaphane_amd.gpu import PinnedBuffer
+
+        pinned = PinnedBuffer(3 * cells)
+        host_e = pinned.array
 
     def run(count: int) -> None:
         if not readback:

@@ -58,6 +58,8 @@ def _declare(lib: ctypes.CDLL) -> None:
     table = {
         "dp_last_error": (c_char_p, []),
         "dp_version": (c_char_p, []),
+        "dp_host_alloc": (c_int, [c_size_t, POINTER(c_void_p)]),
+        "dp_host_free": (None, [c_void_p]),
         "dp_grid_resolve": (c_int, [POINTER(DpGridSpec), POINTER(c_uint32), POINTER(c_float), c_size_t,
                                     POINTER(c_float)]),
         "dp_scene_validate": (c_int, [POINTER(DpScene)]),

@@ -39,6 +39,18 @@ extern "C" {
 const char *dp_last_error(void) { return last_error(); }
 const char *dp_version(void) { return DPA_VERSION; }
 
+int dp_host_alloc(size_t bytes, void **out) {
+    return guarded([&] {
+        if (!out || bytes == 0) fail(DP_ERR_INVALID, "bad argument");
+        *out = nullptr;
+        check(hipHostMalloc(out, bytes, hipHostMallocDefault), "hipHostMalloc");
+    });
+}
+
+void dp_host_free(void *ptr) {
+    if (ptr) (void)hipHostFree(ptr);
+}
+
 int dp_grid_resolve(const dp_grid_spec *spec, uint32_t extent_out[3], float *widths_out,
                     size_t capacity, float *time_step_out) {
     return guarded([&] {

@@ -80,6 +80,28 @@ def headless_context(device: int = 0) -> Context:
     return _default_context
 
 
+class PinnedBuffer:
+    """Page-locked host memory from the engine (dp_host_alloc), freed when
+    this object goes; `array` views it."""
+
+    def __init__(self, count: int, dtype=np.float32):
+        dtype = np.dtype(dtype)
+        ptr = c_void_p()
+        check(lib().dp_host_alloc(int(count) * dtype.itemsize, byref(ptr)))
+        self._ptr = ptr
+        raw = (ctypes.c_char * (int(count) * dtype.itemsize)).from_address(ptr.value)
+        self.array = np.frombuffer(raw, dtype=dtype, count=int(count))
+
+    def __del__(self):
+        ptr = getattr(self, "_ptr", None)
+        if ptr:
+            try:
+                lib().dp_host_free(ptr)
+            except Exception:
+                pass
+            self._ptr = None
+
+
 def _fptr(array: np.ndarray):
     return array.ctypes.data_as(POINTER(c_float))
 

@@ -164,6 +164,13 @@ const char *dp_last_error(void);
 /* Library version string. */
 const char *dp_version(void);
 
+/* Page-locked host memory for repeated readbacks (dp_sim_read_*,
+ * dp_sim_write_fields): copies to and from it run at full PCIe rate with no
+ * staging. No counterpart in the reference, whose readbacks map its own
+ * staging buffers. dp_host_free(NULL) does nothing. */
+int dp_host_alloc(size_t bytes, void **out);
+void dp_host_free(void *ptr);
+
 /* Resolves a GridSpec: Grid::build (src/grid.rs:466-500, grade() at
  * :791-888). extent_out gets the resolved cell counts; widths_out (may be
  * NULL) gets the primary widths of x, then y, then z

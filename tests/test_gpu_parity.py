@@ -321,3 +321,18 @@ def test_readback_into_a_reused_host_buffer(context):
         with pytest.raises(DiaphaneError):
             sim.read_electric(out=wrong)
     sim.close()
+
+
+def test_readback_into_pinned_host_memory(context):
+    import numpy as np
+    from diaphane_amd import Extent, Scene
+    from diaphane_amd.gpu import PinnedBuffer, Simulation
+
+    sim = Simulation(context, Scene.photon(Extent(40, 32, 30)))
+    sim.advance_by(9)
+    pinned = PinnedBuffer(3 * sim.cells)
+    assert np.array_equal(sim.read_electric(out=pinned.array).view(np.uint32), sim.read_electric().view(np.uint32))
+    sim.write_fields(pinned.array, sim.read_magnetic(), sim.step_count())
+    sim.advance_by(3)
+    del pinned
+    sim.close()
