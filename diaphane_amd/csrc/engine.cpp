@@ -467,6 +467,13 @@ int dp_sim_device_ptr(dp_sim *sim, int which, void **out) {
         case DP_BUFFER_ELECTRIC: *out = sim->e[sim->current].ptr; break;
         case DP_BUFFER_MAGNETIC: *out = sim->h[sim->current].ptr; break;
         case DP_BUFFER_INTENSITY: sim->ensure_intensity(); *out = sim->intensity.ptr; break;
+        case DP_BUFFER_INTENSITY_F32:
+            if (!sim->intensity_f32.ptr) sim->intensity_f32.allocate(sim->cells * sizeof(float));
+            launch_intensity_f32(sim->intensity.ptr ? sim->intensity.as<double>() : nullptr,
+                                 sim->intensity_f32.as<float>(), sim->cells, sim->stream);
+            check(hipGetLastError(), "kernel launch");
+            *out = sim->intensity_f32.ptr;
+            break;
         case DP_BUFFER_MATERIAL_INDEX: *out = sim->material.ptr; break;
         default: fail(DP_ERR_INVALID, "unknown buffer");
         }

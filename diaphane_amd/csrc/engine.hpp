@@ -125,6 +125,7 @@ struct dp_sim {
     DeviceBuffer material, coefficients, axis_table, weights, eps_mu, widths;
     DeviceBuffer intensity, injections, partials;
     DeviceBuffer peak_partials;  // per-block maxima of dp_sim_measure_peak
+    DeviceBuffer intensity_f32;  // DP_BUFFER_INTENSITY_F32, refreshed on request
     PinnedBuffer staging;
     int current = 0;  // which of the ping-pong pair holds the state
 

@@ -369,6 +369,12 @@ __global__ __launch_bounds__(256) void k_inject(float *field, uint32_t nx, uint3
     field[i] = field[i] + d.value * w;
 }
 
+// The f64 intensity sum rounded to f32, for a renderer that binds f32.
+__global__ __launch_bounds__(256) void k_intensity_f32(const double *sum, float *out, uint64_t n) {
+    const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = sum ? float(sum[i]) : 0.0f;
+}
+
 __global__ __launch_bounds__(256) void k_intensity(const float *e, double *intensity, uint64_t n) {
     const uint64_t i = uint64_t(blockIdx.x) * 256 + threadIdx.x;
     if (i >= n) return;
@@ -743,6 +749,11 @@ void launch_inject(float *field, uint64_t cells, uint32_t nx, uint32_t ny, const
                    const InjDesc *, const float *weights, hipStream_t stream) {
     dim3 grid((desc.extent[0] + 255) / 256, desc.extent[1], desc.extent[2]);
     k_inject<<<grid, 256, 0, stream>>>(field + uint64_t(desc.component) * cells, nx, ny, desc, weights);
+}
+
+void launch_intensity_f32(const double *sum, float *out, uint64_t cells, hipStream_t stream) {
+    const uint32_t blocks = uint32_t((cells + 255) / 256);
+    k_intensity_f32<<<blocks, 256, 0, stream>>>(sum, out, cells);
 }
 
 void launch_intensity(const float *e, double *intensity, uint64_t cells, hipStream_t stream) {

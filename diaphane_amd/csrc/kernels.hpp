@@ -132,6 +132,8 @@ void launch_update_e(const StepArgs &args, int direction, hipStream_t stream);
 void launch_inject(float *field, uint64_t cells, uint32_t nx, uint32_t ny, const InjDesc &desc,
                    const InjDesc *device_desc, const float *weights, hipStream_t stream);
 void launch_intensity(const float *e, double *intensity, uint64_t cells, hipStream_t stream);
+// out[i] = the f32 rounding of sum[i] (0 where sum is null).
+void launch_intensity_f32(const double *sum, float *out, uint64_t cells, hipStream_t stream);
 // f64 energy partials, one pair per block; returns the number of blocks.
 uint32_t launch_energy(const float *e, const float *h, const uint8_t *material,
                        const float2 *eps_mu, const float *widths, uint32_t nx, uint32_t ny,

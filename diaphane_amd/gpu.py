@@ -29,6 +29,7 @@ BUFFER_ELECTRIC = 0
 BUFFER_MAGNETIC = 1
 BUFFER_INTENSITY = 2
 BUFFER_MATERIAL_INDEX = 3
+BUFFER_INTENSITY_F32 = 4
 
 
 @dataclass

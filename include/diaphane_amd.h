@@ -295,9 +295,13 @@ int dp_sim_energy(dp_sim *sim, double *electric, double *magnetic);
  * 367-374, 597-599): device pointers, borrowed. The engine ping-pongs
  * E and H between two allocations each, so the pointer names the buffer
  * holding the current state and stays valid until the next advance,
- * reverse, reset or write. Intensity is f64. */
+ * reverse, reset or write. DP_BUFFER_INTENSITY is the f64 sum. For a
+ * renderer that binds f32 like the reference's intensity_buffer,
+ * DP_BUFFER_INTENSITY_F32 rounds the sum to f32 into a buffer of its own,
+ * ordered on the sim's stream at this call: the values dp_sim_read_intensity
+ * returns, as of this call. */
 enum { DP_BUFFER_ELECTRIC = 0, DP_BUFFER_MAGNETIC = 1, DP_BUFFER_INTENSITY = 2,
-       DP_BUFFER_MATERIAL_INDEX = 3 };
+       DP_BUFFER_MATERIAL_INDEX = 3, DP_BUFFER_INTENSITY_F32 = 4 };
 int dp_sim_device_ptr(dp_sim *sim, int which, void **out);
 
 /* Kernel path for the forward step. FUSED (default): one z-marching pass
