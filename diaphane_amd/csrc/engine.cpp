@@ -211,7 +211,7 @@ int dp_sim_create(dp_engine *engine, const dp_scene *scene, dp_sim **out) {
         if (sim->cells >= dp_sim::kTwoStepDefaultCells &&
             buffer_addressing_fits(sim->cells, g.extent[0], g.extent[1])) {
             sim->tiling[0] = {10, 0, DP_TILE_TWO_STEP | DP_TILE_BUFFER};
-            sim->tiling[1] = {6, 0, DP_TILE_TWO_STEP | DP_TILE_BUFFER};
+            sim->tiling[1] = {10, 0, DP_TILE_TWO_STEP | DP_TILE_BUFFER};
         }
         sim->time_step = g.time_step();
         sim->material_count = scene->material_count;

@@ -331,9 +331,8 @@ int dp_sim_set_kernel(dp_sim *sim, int kernel);
  *                         other values select 8); an odd step left over in
  *                         an advance runs single.
  * A new sim of at least 32 Mi cells whose offsets fit buffer addressing
- * starts with two steps per launch (10 rows for plain steps, 6 with the
- * intensity sum on, buffer addressing); smaller ones with one step (4 rows,
- * non-temporal stores).
+ * starts with two steps per launch (10 rows, buffer addressing); smaller
+ * ones with one step (4 rows, non-temporal stores).
  * 0 for rows selects 4; 0 for depth selects a size-aware default (the power
  * of two at or below the number of x-y tiles / 50, within 1..32, or / 25
  * within 2..32 for two steps per launch: short slabs give a small grid
