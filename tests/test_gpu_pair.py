@@ -166,3 +166,30 @@ def test_random_scenes_in_pairs_match_the_oracle(context, seed):
     assert np.abs(reference.electric_all).max() > 0.0, "nothing was injected"
     assert_same(reference, device, case.describe(), intensity)
     device.close()
+
+
+def test_large_grids_default_to_two_steps_per_launch(context):
+    # At 32 Mi cells and above (with buffer addressing) a new sim starts two
+    # steps per launch; its fields equal one step per launch bit for bit.
+    scene = Scene.photon(Extent(320, 320, 328))  # 33.6M cells
+    device = Simulation(context, scene)
+    assert device.cells >= 32 << 20
+    before = device.launch_count()
+    device.advance_by(5)
+    assert device.launch_count() - before == 3  # two pairs and a single step
+    single = Simulation(context, scene)
+    single.set_tile(4, 0, TILE_NONTEMPORAL)
+    single.advance_by(5)
+    assert differing(device.read_electric(), single.read_electric()) == 0
+    assert differing(device.read_magnetic(), single.read_magnetic()) == 0
+    for sim in (device, single):
+        sim.set_accumulate_intensity(True)
+        sim.advance_by(4)
+    assert differing(device.read_intensity_f64(), single.read_intensity_f64()) == 0
+    device.close()
+    single.close()
+    small = Simulation(context, Scene.photon(Extent(96, 64, 64)))
+    before = small.launch_count()
+    small.advance_by(5)
+    assert small.launch_count() - before == 5  # one step per launch below the threshold
+    small.close()
