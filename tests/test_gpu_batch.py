@@ -202,7 +202,7 @@ def test_autotuning_leaves_members_unchanged(context):
         batch.add(sim)
     batch.advance_by(9)
     rows, depth, ms = batch.autotune(2)
-    assert rows in (4, 6, 8) and depth in (8, 16, 32, 64) and ms > 0.0
+    assert rows in (4, 6, 8, 10) and depth in (8, 16, 32, 64) and ms > 0.0  # 10: two steps per launch
     batch.advance_by(11)
     batch.wait()
     for sim, instance in zip(members, instances):
