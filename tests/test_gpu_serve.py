@@ -71,7 +71,7 @@ def test_results_read_while_other_requests_run(context):
     # capturing and launching graphs for the others: the library's copies
     # are stream-ordered, never on the legacy stream (which HIP refuses
     # while a stream is capturing).
-    stream = served_stream(8, rate=200.0, seed=25, lo=32, hi=48, steps_lo=40, steps_hi=400)
+    stream = served_stream(8, rate=200.0, seed=25, lo=32, hi=48, steps_lo=40, steps_hi=1200)
     alone = [solo_fields(context, r) for r in stream]
     with Server(context, chunk_steps=8) as server:
         tickets = {server.submit(r.scene, r.steps): k for k, r in enumerate(stream)}
