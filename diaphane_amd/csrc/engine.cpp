@@ -269,6 +269,8 @@ int dp_sim_create(dp_engine *engine, const dp_scene *scene, dp_sim **out) {
 
         check(hipStreamCreateWithFlags(&sim->stream, hipStreamNonBlocking), "hipStreamCreate");
         for (auto &ev : sim->events) check(hipEventCreate(&ev), "hipEventCreate");
+        for (auto &slot : sim->injection_slots)
+            check(hipEventCreateWithFlags(&slot.done, hipEventDisableTiming), "hipEventCreateWithFlags");
         const size_t field_bytes = 3 * sim->cells * sizeof(float);
         for (int b = 0; b < 2; ++b) {
             sim->e[b].allocate(field_bytes);
@@ -283,7 +285,7 @@ int dp_sim_create(dp_engine *engine, const dp_scene *scene, dp_sim **out) {
         upload(sim->widths, widths, sim->stream);
         upload(sim->weights, weights, sim->stream);
         sim->partials.allocate(2 * sizeof(double) * energy_blocks(g.extent[0], g.extent[1], g.extent[2]));
-        sim->injections.allocate(64 * sizeof(InjDesc));
+        sim->empty_injections.allocate(sizeof(InjDesc));
         sim->sync();
         *out = sim.release();
     });
@@ -521,7 +523,7 @@ int dp_sim_autotune(dp_sim *sim, uint32_t trials, int *rows, int *depth, int *fl
         // intensity is timed with the read-modify-write going to a scratch
         // buffer. The state is unchanged.
         StepArgs a = sim->args(sim->current, 1 - sim->current);
-        a.injections = sim->injections.as<InjDesc>();
+        a.injections = sim->empty_injections.as<InjDesc>();
         a.n_magnetic = a.n_electric = 0;
         DeviceBuffer scratch;
         a.intensity = nullptr;

@@ -123,10 +123,24 @@ struct dp_sim {
     hipStream_t stream = nullptr;
     DeviceBuffer e[2], h[2];
     DeviceBuffer material, coefficients, axis_table, weights, eps_mu, widths;
-    DeviceBuffer intensity, injections, partials;
+    DeviceBuffer intensity, partials;
+    DeviceBuffer empty_injections;  // the autotuner's trial launches inject nothing
     DeviceBuffer peak_partials;  // per-block maxima of dp_sim_measure_peak
     DeviceBuffer intensity_f32;  // DP_BUFFER_INTENSITY_F32, refreshed on request
-    PinnedBuffer staging;
+    // Injection tables of the advances in flight. Each advance fills the next
+    // slot's pinned staging, copies it to the slot's device table and records
+    // `done` after its last launch, so an advance waits only for the one that
+    // used its slot kInjectionSlots advances earlier (long finished, unless
+    // the host runs that far ahead), never for the previous one.
+    struct InjectionSlot {
+        PinnedBuffer staging;
+        DeviceBuffer table;
+        hipEvent_t done = nullptr;
+        bool pending = false;
+    };
+    static constexpr int kInjectionSlots = 4;
+    InjectionSlot injection_slots[kInjectionSlots];
+    int next_injection_slot = 0;
     int current = 0;  // which of the ping-pong pair holds the state
 
     uint64_t step = 0;
@@ -154,6 +168,8 @@ struct dp_sim {
         if (stream) (void)hipStreamSynchronize(stream);
         for (auto &ev : events)
             if (ev) (void)hipEventDestroy(ev);
+        for (auto &slot : injection_slots)
+            if (slot.done) (void)hipEventDestroy(slot.done);
         if (stream) (void)hipStreamDestroy(stream);
     }
 
@@ -242,15 +258,31 @@ struct dp_sim {
         return table;
     }
 
-    void upload_injections(const std::vector<InjDesc> &all) {
+    // Fills the next injection slot with `all` and queues its copy to the
+    // device; the caller launches from the slot's table and then calls
+    // release_injections.
+    InjectionSlot &upload_injections(const std::vector<InjDesc> &all) {
+        InjectionSlot &slot = injection_slots[next_injection_slot];
+        next_injection_slot = (next_injection_slot + 1) % kInjectionSlots;
+        if (slot.pending) check(hipEventSynchronize(slot.done), "hipEventSynchronize");
+        slot.pending = false;
         const size_t bytes = std::max<size_t>(all.size(), 1) * sizeof(InjDesc);
-        // The previous advance may still read the staging or device table.
-        sync();
-        staging.reserve(bytes);
-        if (!all.empty()) std::memcpy(staging.ptr, all.data(), all.size() * sizeof(InjDesc));
-        if (injections.bytes < bytes) injections.allocate(bytes * 2);
-        check(hipMemcpyAsync(injections.ptr, staging.ptr, bytes, hipMemcpyHostToDevice, stream),
+        slot.staging.reserve(bytes);
+        if (!all.empty()) std::memcpy(slot.staging.ptr, all.data(), all.size() * sizeof(InjDesc));
+        if (slot.table.bytes < bytes) slot.table.allocate(bytes * 2);
+        check(hipMemcpyAsync(slot.table.ptr, slot.staging.ptr, bytes, hipMemcpyHostToDevice, stream),
               "hipMemcpyAsync");
+        return slot;
+    }
+
+    // Marks the slot busy until the stream passes this point (after the
+    // launches that read its table). Should the record fail, the stream is
+    // drained instead, so the slot is free for its next use either way.
+    void release_injections(InjectionSlot &slot) {
+        if (hipEventRecord(slot.done, stream) == hipSuccess)
+            slot.pending = true;
+        else
+            (void)hipStreamSynchronize(stream);
     }
 
     // The default z-slab depth for a grid whose x-y plane holds `tiles_xy`
@@ -321,8 +353,13 @@ struct dp_sim {
         if (steps == 0) return;
         std::vector<InjDesc> all;
         auto table = build_injections(step, steps, +1, all);
-        upload_injections(all);
-        const InjDesc *device_inj = injections.as<InjDesc>();
+        InjectionSlot &slot = upload_injections(all);
+        struct Release {
+            dp_sim *sim;
+            InjectionSlot &slot;
+            ~Release() { sim->release_injections(slot); }
+        } release{this, slot};
+        const InjDesc *device_inj = slot.table.as<InjDesc>();
         if (accumulating) ensure_intensity();
         const int r = effective_rows();
         const bool pairs = kernel == DP_KERNEL_FUSED && (active().flags & DP_TILE_TWO_STEP);
@@ -381,8 +418,9 @@ struct dp_sim {
         if (step < steps) fail(DP_ERR_PAST_START, "cannot step back past the start");
         clear_intensity();
         std::vector<InjDesc> all;
+        // The injection kernels take their descriptors by value, so no
+        // device table is needed here.
         auto table = build_injections(step, steps, -1, all);
-        upload_injections(all);
         for (uint64_t s = 0; s < steps; ++s) {
             const auto &t = table[s];
             StepArgs a = args(current, current);
