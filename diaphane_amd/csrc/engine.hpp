@@ -124,7 +124,7 @@ struct dp_sim {
     DeviceBuffer e[2], h[2];
     DeviceBuffer material, coefficients, axis_table, weights, eps_mu, widths;
     DeviceBuffer intensity, partials;
-    DeviceBuffer empty_injections;  // the autotuner's trial launches inject nothing
+    DeviceBuffer empty_injections;  // one entry, for launches with no injections
     DeviceBuffer peak_partials;  // per-block maxima of dp_sim_measure_peak
     DeviceBuffer intensity_f32;  // DP_BUFFER_INTENSITY_F32, refreshed on request
     // Injection tables of the advances in flight. Each advance fills the next
@@ -260,27 +260,35 @@ struct dp_sim {
 
     // Fills the next injection slot with `all` and queues its copy to the
     // device; the caller launches from the slot's table and then calls
-    // release_injections.
-    InjectionSlot &upload_injections(const std::vector<InjDesc> &all) {
+    // release_injections. An empty list takes no slot (nullptr): it copies
+    // nothing and launches from the one-entry `empty_injections`, which no
+    // kernel reads with zero counts.
+    InjectionSlot *upload_injections(const std::vector<InjDesc> &all, const InjDesc *&device_table) {
+        if (all.empty()) {
+            device_table = empty_injections.as<InjDesc>();
+            return nullptr;
+        }
         InjectionSlot &slot = injection_slots[next_injection_slot];
         next_injection_slot = (next_injection_slot + 1) % kInjectionSlots;
         if (slot.pending) check(hipEventSynchronize(slot.done), "hipEventSynchronize");
         slot.pending = false;
-        const size_t bytes = std::max<size_t>(all.size(), 1) * sizeof(InjDesc);
+        const size_t bytes = all.size() * sizeof(InjDesc);
         slot.staging.reserve(bytes);
-        if (!all.empty()) std::memcpy(slot.staging.ptr, all.data(), all.size() * sizeof(InjDesc));
+        std::memcpy(slot.staging.ptr, all.data(), bytes);
         if (slot.table.bytes < bytes) slot.table.allocate(bytes * 2);
         check(hipMemcpyAsync(slot.table.ptr, slot.staging.ptr, bytes, hipMemcpyHostToDevice, stream),
               "hipMemcpyAsync");
-        return slot;
+        device_table = slot.table.as<InjDesc>();
+        return &slot;
     }
 
     // Marks the slot busy until the stream passes this point (after the
     // launches that read its table). Should the record fail, the stream is
     // drained instead, so the slot is free for its next use either way.
-    void release_injections(InjectionSlot &slot) {
-        if (hipEventRecord(slot.done, stream) == hipSuccess)
-            slot.pending = true;
+    void release_injections(InjectionSlot *slot) {
+        if (!slot) return;
+        if (hipEventRecord(slot->done, stream) == hipSuccess)
+            slot->pending = true;
         else
             (void)hipStreamSynchronize(stream);
     }
@@ -353,13 +361,12 @@ struct dp_sim {
         if (steps == 0) return;
         std::vector<InjDesc> all;
         auto table = build_injections(step, steps, +1, all);
-        InjectionSlot &slot = upload_injections(all);
+        const InjDesc *device_inj = nullptr;
         struct Release {
             dp_sim *sim;
-            InjectionSlot &slot;
+            InjectionSlot *slot;
             ~Release() { sim->release_injections(slot); }
-        } release{this, slot};
-        const InjDesc *device_inj = slot.table.as<InjDesc>();
+        } release{this, upload_injections(all, device_inj)};
         if (accumulating) ensure_intensity();
         const int r = effective_rows();
         const bool pairs = kernel == DP_KERNEL_FUSED && (active().flags & DP_TILE_TWO_STEP);
