@@ -249,7 +249,10 @@ uint64_t dp_sim_bytes_per_step(const dp_sim *sim);
 int dp_sim_is_reversible(const dp_sim *sim);
 
 /* advance_by (src/gpu.rs:393-484): enqueues `steps` full steps on the
- * sim's stream and returns without waiting. */
+ * sim's stream and returns without waiting. The call's injection table is
+ * copied through one of four event-guarded slots, so back-to-back calls queue
+ * on the stream; a call blocks only while the device is still four
+ * table-carrying calls behind. */
 int dp_sim_advance(dp_sim *sim, uint64_t steps);
 /* reverse_by (src/gpu.rs:499-576): DP_ERR_LOSSY / DP_ERR_PAST_START. */
 int dp_sim_reverse(dp_sim *sim, uint64_t steps);
