@@ -79,7 +79,8 @@ class Batch:
         check(lib().dp_batch_tiling(self._handle, byref(rows), byref(depth), byref(flags)))
         return {"rows": rows.value, "depth": depth.value, "wave_per_tile": bool(flags.value & WAVE_PER_TILE),
                 "buffer": bool(flags.value & 4), "nontemporal": bool(flags.value & 2),
-                "two_step": bool(flags.value & 8)}
+                "two_step": bool(flags.value & 8),
+                "xcd_group": (((flags.value >> 8) & 0xFF) or 8) if flags.value & 1 else 0}
 
     def __len__(self) -> int:
         return int(lib().dp_batch_size(self._handle))

@@ -113,10 +113,11 @@ bool buffer_addressing_fits(uint64_t cells, uint32_t nx, uint32_t ny);
 constexpr int kPairOwnedX = kLanes - 4;
 void launch_fused_pair(const StepArgs &args, int rows, bool nontemporal, bool buffer, hipStream_t stream);
 // Many grids, two steps per launch: one wave per tile of the packed table
-// (tiles cut for the two-step kernel).
+// (tiles cut for the two-step kernel). `xcd_group` > 0 gives each XCD runs of
+// that many consecutive workgroups; 0 keeps the hardware's round robin.
 void launch_fused_pair_batch(const InstanceDesc *instances, const StepSlot *slots, const uint32_t *owners,
                              uint32_t n_tiles, const InjDesc *injections, int rows, bool nontemporal, bool buffer,
-                             hipStream_t stream);
+                             uint32_t xcd_group, hipStream_t stream);
 // One launch advancing every active instance of a batch by one step.
 // `blocks` is the persistent grid size (workgroups of 4 waves); with
 // `direct` the launch instead has one wave per tile.

@@ -56,23 +56,24 @@ void launch_fused_pair(const StepArgs &args, int rows, bool nontemporal, bool bu
 template <template <bool> class Access>
 static void launch_pair_batch_as(const InstanceDesc *instances, const StepSlot *slots, const uint32_t *owners,
                                  uint32_t n_tiles, const InjDesc *injections, int rows, bool nontemporal,
-                                 hipStream_t stream) {
+                                 uint32_t xcd_group, hipStream_t stream) {
     const uint32_t blocks = (n_tiles + 3) / 4;
 #define DPA_ROWS(R)                                                                                             \
     case R:                                                                                                     \
         if (nontemporal)                                                                                        \
             k_fused_pair_batch<R, Access<true>><<<blocks, 256, 0, stream>>>(instances, slots, owners, n_tiles,   \
-                                                                           injections);                         \
+                                                                           injections, xcd_group);              \
         else                                                                                                    \
             k_fused_pair_batch<R, Access<false>><<<blocks, 256, 0, stream>>>(instances, slots, owners, n_tiles,  \
-                                                                            injections);                        \
+                                                                            injections, xcd_group);             \
         break;
     switch (rows) {
         DPA_ROWS(6)
         DPA_ROWS(8)
         DPA_ROWS(10)
     default:
-        k_fused_pair_batch<8, Access<true>><<<blocks, 256, 0, stream>>>(instances, slots, owners, n_tiles, injections);
+        k_fused_pair_batch<8, Access<true>><<<blocks, 256, 0, stream>>>(instances, slots, owners, n_tiles, injections,
+                                                                    xcd_group);
         break;
     }
 #undef DPA_ROWS
@@ -80,12 +81,14 @@ static void launch_pair_batch_as(const InstanceDesc *instances, const StepSlot *
 
 void launch_fused_pair_batch(const InstanceDesc *instances, const StepSlot *slots, const uint32_t *owners,
                              uint32_t n_tiles, const InjDesc *injections, int rows, bool nontemporal, bool buffer,
-                             hipStream_t stream) {
+                             uint32_t xcd_group, hipStream_t stream) {
     if (n_tiles == 0) return;
     if (buffer)
-        launch_pair_batch_as<BufferAccess>(instances, slots, owners, n_tiles, injections, rows, nontemporal, stream);
+        launch_pair_batch_as<BufferAccess>(instances, slots, owners, n_tiles, injections, rows, nontemporal, xcd_group,
+                                           stream);
     else
-        launch_pair_batch_as<Pointer64>(instances, slots, owners, n_tiles, injections, rows, nontemporal, stream);
+        launch_pair_batch_as<Pointer64>(instances, slots, owners, n_tiles, injections, rows, nontemporal, xcd_group,
+                                        stream);
 }
 
 }  // namespace dpa

@@ -444,8 +444,11 @@ int dp_batch_set_graph_steps(dp_batch *batch, uint32_t steps);
  * launches one wave per tile instead of the persistent grid whose waves
  * stride over the tile list, and DP_TILE_TWO_STEP: each launch advances
  * every member two steps (one wave per tile; a member's last step of an odd
- * count runs alone in its launch). The default is two steps per launch, 6
- * rows, DP_TILE_BUFFER, cached stores. */
+ * count runs alone in its launch). With DP_TILE_TWO_STEP, DP_TILE_XCD_REMAP
+ * (run length G in bits 8..15, 0 meaning 8) gives each XCD runs of G
+ * consecutive workgroups of the tile list; the one-step batch kernels ignore
+ * it. The default is two steps per launch, 6 rows, DP_TILE_BUFFER, cached
+ * stores, XCD runs of 4. The autotuner keeps the XCD bits as they are. */
 enum { DP_BATCH_WAVE_PER_TILE = 16 };
 int dp_batch_set_tile(dp_batch *batch, int rows, int depth, int flags);
 /* Autotuner: times `trials` launches of every grid form × addressing ×
