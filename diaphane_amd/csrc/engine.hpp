@@ -158,6 +158,9 @@ struct dp_sim {
     };
     Tiling tiling[2];
     static constexpr uint64_t kTwoStepDefaultCells = 32ull << 20;
+    // Two steps per launch as the size rule and the autotuner set it: with
+    // XCD-grouped workgroups (DP_TILE_XCD_REMAP, runs of 8).
+    static constexpr int kTwoStepTileFlags = DP_TILE_TWO_STEP | DP_TILE_XCD_REMAP;
     const Tiling &active() const { return tiling[accumulating ? 1 : 0]; }
     uint64_t launches = 0;
     hipEvent_t events[8] = {};
