@@ -62,6 +62,8 @@ def _declare(lib: ctypes.CDLL) -> None:
         "dp_host_free": (None, [c_void_p]),
         "dp_grid_resolve": (c_int, [POINTER(DpGridSpec), POINTER(c_uint32), POINTER(c_float), c_size_t,
                                     POINTER(c_float)]),
+        "dp_grid_cell_lookup": (c_int, [POINTER(DpGridSpec), c_uint32, POINTER(c_float), c_size_t]),
+        "dp_grid_lookup_samples": (c_int, [POINTER(DpGridSpec), POINTER(c_uint32)]),
         "dp_scene_validate": (c_int, [POINTER(DpScene)]),
         "dp_scene_from_ron": (c_int, [c_char_p, POINTER(c_void_p)]),
         "dp_scene_file_scene": (POINTER(DpScene), [c_void_p]),
@@ -101,6 +103,7 @@ def _declare(lib: ctypes.CDLL) -> None:
         "dp_sim_record_event": (c_int, [sim, c_int]),
         "dp_sim_elapsed_ms": (c_int, [sim, c_int, c_int, POINTER(c_float)]),
         "dp_sim_launch_count": (c_uint64, [sim]),
+        "dp_sim_lookup_samples": (c_uint32, [sim]),
         "dp_sim_stream_probe": (c_int, [sim, c_int, POINTER(c_double)]),
         "dp_sim_tile_probe": (c_int, [sim, c_int, c_int, c_int, POINTER(c_double)]),
         "dp_sim_measure_peak": (c_int, [sim, POINTER(c_float)]),

@@ -33,6 +33,20 @@ struct StagingSet {
         graph = nullptr;
         graph_steps = 0;
     }
+
+    // Makes room for a chunk of `slot_bytes` of slots and `inj_bytes` of
+    // injections. The caller has waited for this set's previous chunk, so
+    // no copy or launch reads these buffers. They only grow, with headroom,
+    // and a graph captured from this set (it holds the device addresses as
+    // kernel arguments) is destroyed before any buffer it names is freed.
+    void reserve(size_t slot_bytes, size_t inj_bytes) {
+        const bool grow_slots = slots.bytes < slot_bytes;
+        const bool grow_injections = injections.bytes < inj_bytes;
+        if (grow_slots || grow_injections) drop_graph();
+        if (grow_slots) slots.allocate(slot_bytes * 2);
+        if (grow_injections) injections.allocate(inj_bytes * 2);
+        host.reserve(slot_bytes + inj_bytes);
+    }
 };
 
 }  // namespace
@@ -190,8 +204,14 @@ struct dp_batch {
         if (owner.size() > std::numeric_limits<uint32_t>::max() / 2)
             fail(DP_ERR_UNSUPPORTED, "too many tiles in one batch");
         n_tiles = static_cast<uint32_t>(owner.size());
-        instance_table.allocate(std::max<size_t>(table.size(), 1) * sizeof(InstanceDesc));
-        owners.allocate(std::max<size_t>(owner.size(), 1) * sizeof(uint32_t));
+        // Both sets' graphs name the table and the owner list: they go before
+        // either buffer can be freed (the tables only grow, with headroom).
+        const size_t table_bytes = std::max<size_t>(table.size(), 1) * sizeof(InstanceDesc);
+        const size_t owner_bytes = std::max<size_t>(owner.size(), 1) * sizeof(uint32_t);
+        if (instance_table.bytes < table_bytes || owners.bytes < owner_bytes)
+            for (auto &set : sets) set.drop_graph();
+        instance_table.reserve(table_bytes);
+        owners.reserve(owner_bytes);
         if (!table.empty())
             copy_on(stream, instance_table.ptr, table.data(), table.size() * sizeof(InstanceDesc), hipMemcpyHostToDevice);
         if (!owner.empty())
@@ -214,7 +234,7 @@ struct dp_batch {
             slot[i].active = 1;
             slot[i].parity = static_cast<uint32_t>(members[i]->current);
         }
-        trial_slots.allocate(std::max<size_t>(slot.size(), 1) * sizeof(StepSlot));
+        trial_slots.reserve(std::max<size_t>(slot.size(), 1) * sizeof(StepSlot));
         copy_on(stream, trial_slots.ptr, slot.data(), slot.size() * sizeof(StepSlot), hipMemcpyHostToDevice);
         if (!trial_injections.ptr) trial_injections.allocate(sizeof(InjDesc));
         const bool nt = (flags & DP_TILE_NONTEMPORAL) != 0;
@@ -392,19 +412,11 @@ struct dp_batch {
             }
             const size_t slot_bytes = slot_buf.size() * sizeof(StepSlot);
             const size_t inj_bytes = std::max<size_t>(inj_all.size(), 1) * sizeof(InjDesc);
-            set.host.reserve(slot_bytes + inj_bytes);
+            set.reserve(slot_bytes, inj_bytes);
             std::memcpy(set.host.ptr, slot_buf.data(), slot_bytes);
             if (!inj_all.empty())
                 std::memcpy(static_cast<char *>(set.host.ptr) + slot_bytes, inj_all.data(),
                             inj_all.size() * sizeof(InjDesc));
-            if (set.slots.bytes < slot_bytes) {
-                set.slots.allocate(slot_bytes);
-                set.drop_graph();
-            }
-            if (set.injections.bytes < inj_bytes) {
-                set.injections.allocate(inj_bytes * 2);
-                set.drop_graph();
-            }
             check(hipMemcpyAsync(set.slots.ptr, set.host.ptr, slot_bytes, hipMemcpyHostToDevice, stream),
                   "hipMemcpyAsync");
             check(hipMemcpyAsync(set.injections.ptr, static_cast<char *>(set.host.ptr) + slot_bytes, inj_bytes,

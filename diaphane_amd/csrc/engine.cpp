@@ -68,6 +68,24 @@ int dp_grid_resolve(const dp_grid_spec *spec, uint32_t extent_out[3], float *wid
     });
 }
 
+int dp_grid_cell_lookup(const dp_grid_spec *spec, uint32_t samples, float *out, size_t capacity) {
+    return guarded([&] {
+        if (!spec || !out) fail(DP_ERR_INVALID, "null argument");
+        if (samples < 2) fail(DP_ERR_INVALID, "a lookup needs at least two samples per axis");
+        if (capacity < size_t(3) * samples) fail(DP_ERR_SIZE, "lookup buffer is too small");
+        const Grid g = Grid::resolve(*spec);
+        const std::vector<float> lookup = g.cell_lookup(samples);
+        std::memcpy(out, lookup.data(), lookup.size() * sizeof(float));
+    });
+}
+
+int dp_grid_lookup_samples(const dp_grid_spec *spec, uint32_t *samples) {
+    return guarded([&] {
+        if (!spec || !samples) fail(DP_ERR_INVALID, "null argument");
+        *samples = Grid::resolve(*spec).lookup_samples();
+    });
+}
+
 struct dp_scene_file {
     std::unique_ptr<dpa::SceneFile> file;
 };
@@ -269,6 +287,9 @@ int dp_sim_create(dp_engine *engine, const dp_scene *scene, dp_sim **out) {
 
         std::vector<uint8_t> indices(sim->cells);
         rasterize_u8(*scene, g, indices.data());
+        // The renderer's world-to-cell table (src/gpu.rs:225-228).
+        sim->lookup_samples = g.lookup_samples();
+        const std::vector<float> lookup = g.cell_lookup(sim->lookup_samples);
 
         check(hipStreamCreateWithFlags(&sim->stream, hipStreamNonBlocking), "hipStreamCreate");
         for (auto &ev : sim->events) check(hipEventCreate(&ev), "hipEventCreate");
@@ -287,6 +308,7 @@ int dp_sim_create(dp_engine *engine, const dp_scene *scene, dp_sim **out) {
         upload(sim->axis_table, table, sim->stream);
         upload(sim->widths, widths, sim->stream);
         upload(sim->weights, weights, sim->stream);
+        upload(sim->lookup, lookup, sim->stream);
         sim->partials.allocate(2 * sizeof(double) * energy_blocks(g.extent[0], g.extent[1], g.extent[2]));
         sim->empty_injections.allocate(sizeof(InjDesc));
         sim->sync();
@@ -311,6 +333,7 @@ uint64_t dp_sim_field_bytes(const dp_sim *sim) { return sim ? 3 * sim->cells * s
 uint64_t dp_sim_bytes_per_step(const dp_sim *sim) { return sim ? 4 * dp_sim_field_bytes(sim) : 0; }
 int dp_sim_is_reversible(const dp_sim *sim) { return sim && sim->reversible ? 1 : 0; }
 uint64_t dp_sim_launch_count(const dp_sim *sim) { return sim ? sim->launches : 0; }
+uint32_t dp_sim_lookup_samples(const dp_sim *sim) { return sim ? sim->lookup_samples : 0; }
 
 int dp_sim_advance(dp_sim *sim, uint64_t steps) {
     return guarded([&] {
@@ -480,6 +503,7 @@ int dp_sim_device_ptr(dp_sim *sim, int which, void **out) {
             *out = sim->intensity_f32.ptr;
             break;
         case DP_BUFFER_MATERIAL_INDEX: *out = sim->material.ptr; break;
+        case DP_BUFFER_LOOKUP: *out = sim->lookup.ptr; break;
         default: fail(DP_ERR_INVALID, "unknown buffer");
         }
     });

@@ -72,6 +72,11 @@ struct DeviceBuffer {
         check(hipMalloc(&ptr, n), "hipMalloc");
         bytes = n;
     }
+    // Grows to at least n bytes (twice n when it must grow); the contents
+    // are not kept.
+    void reserve(size_t n) {
+        if (n > bytes) allocate(2 * n);
+    }
     void release() {
         if (ptr) (void)hipFree(ptr);
         ptr = nullptr;
@@ -83,7 +88,9 @@ struct DeviceBuffer {
     }
 };
 
-// Pinned host staging, grown on demand.
+// Pinned host staging, grown on demand with headroom (twice the request),
+// so a run of slowly growing requests re-allocates only a few times. The
+// caller makes sure no pending copy reads the old block before it grows.
 struct PinnedBuffer {
     void *ptr = nullptr;
     size_t bytes = 0;
@@ -94,8 +101,9 @@ struct PinnedBuffer {
         if (n <= bytes) return;
         if (ptr) (void)hipHostFree(ptr);
         ptr = nullptr;
-        check(hipHostMalloc(&ptr, n, hipHostMallocDefault), "hipHostMalloc");
-        bytes = n;
+        bytes = 0;
+        check(hipHostMalloc(&ptr, 2 * n, hipHostMallocDefault), "hipHostMalloc");
+        bytes = 2 * n;
     }
 };
 
@@ -127,6 +135,8 @@ struct dp_sim {
     DeviceBuffer empty_injections;  // one entry, for launches with no injections
     DeviceBuffer peak_partials;  // per-block maxima of dp_sim_measure_peak
     DeviceBuffer intensity_f32;  // DP_BUFFER_INTENSITY_F32, refreshed on request
+    DeviceBuffer lookup;         // DP_BUFFER_LOOKUP: Grid::cell_lookup, 3 × lookup_samples
+    uint32_t lookup_samples = 0;
     // Injection tables of the advances in flight. Each advance fills the next
     // slot's pinned staging, copies it to the slot's device table and records
     // `done` after its last launch, so an advance waits only for the one that

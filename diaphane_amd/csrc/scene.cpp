@@ -238,6 +238,26 @@ void Grid::cell_containing(const float position[3], uint32_t out[3]) const {
     }
 }
 
+std::vector<float> Grid::cell_lookup(uint32_t samples) const {
+    std::vector<float> out;
+    out.reserve(size_t(3) * samples);
+    const float last = static_cast<float>(samples - 1);
+    for (int a = 0; a < 3; ++a) {
+        const float length = axis[a].length();
+        for (uint32_t s = 0; s < samples; ++s) {
+            const float fraction = static_cast<float>(s) / last;
+            out.push_back(axis[a].coordinate((fraction - 0.5f) * length));
+        }
+    }
+    return out;
+}
+
+uint32_t Grid::lookup_samples() const {
+    const uint32_t longest = std::max(extent[0], std::max(extent[1], extent[2]));
+    const uint64_t four = 4ull * longest;
+    return static_cast<uint32_t>(std::min<uint64_t>(std::max<uint64_t>(four, 256), 8192));
+}
+
 bool Grid::contains(const float position[3]) const {
     for (int a = 0; a < 3; ++a)
         if (!(std::fabs(position[a]) <= 0.5f * axis[a].length())) return false;

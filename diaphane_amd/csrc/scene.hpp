@@ -60,6 +60,13 @@ struct Grid {
     float time_step() const;
     std::vector<float> gains(const std::vector<float> &widths) const;
     void cell_containing(const float position[3], uint32_t out[3]) const;
+    // Grid::cell_lookup (src/grid.rs:635-646): per axis x, y, z, `samples`
+    // fractional cell coordinates of positions spread evenly from one wall
+    // to the other (samples >= 2).
+    std::vector<float> cell_lookup(uint32_t samples) const;
+    // The sample count gpu::Simulation::new picks (src/gpu.rs:225-227): four
+    // per cell of the longest axis, within 256..8192.
+    uint32_t lookup_samples() const;
     bool contains(const float position[3]) const;
     float coarsest() const;
 };

@@ -209,16 +209,28 @@ struct dp_server {
             std::vector<Request *> done;
             std::vector<Request *> still;
             for (Request *r : running) (r->sim->step >= r->steps || status != DP_OK ? done : still).push_back(r);
-            for (Request *r : done)
-                if (!config.one_at_a_time) (void)dp_batch_remove(batch, r->sim);
+            // A finished sim leaves the batch here, on the loop thread, so a
+            // client that releases it later never touches the batch. Should
+            // the removal fail, the request fails with that error, and
+            // finish_locked destroys its sim on this thread too.
+            std::vector<int> done_status(done.size(), status);
+            std::vector<std::string> done_error(done.size(), error);
+            for (size_t k = 0; k < done.size(); ++k) {
+                if (config.one_at_a_time) continue;
+                const int removed = dp_batch_remove(batch, done[k]->sim);
+                if (removed != DP_OK && done_status[k] == DP_OK) {
+                    done_status[k] = removed;
+                    done_error[k] = last_error();
+                }
+            }
             std::lock_guard<std::mutex> lock(mu);
             ++chunks;
             busy_seconds += busy;
             if (status != DP_OK && loop_error.empty()) loop_error = error;
             for (Request *r : still) r->step = r->sim->step;
-            for (Request *r : done) {
-                r->step = r->sim->step;
-                finish_locked(r, status, error);
+            for (size_t k = 0; k < done.size(); ++k) {
+                done[k]->step = done[k]->sim->step;
+                finish_locked(done[k], done_status[k], done_error[k]);
             }
             running.swap(still);
             running_count = static_cast<uint32_t>(running.size());

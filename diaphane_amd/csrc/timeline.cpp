@@ -29,13 +29,29 @@ struct dp_timeline {
     };
     std::deque<Frame> ring;  // oldest first
 
-    ~dp_timeline() {
+    ~dp_timeline() { drop_slots(); }
+
+    // Waits until no copy reads or fills a slot, then frees every slot and
+    // its buffer.
+    void drop_slots() noexcept {
         for (Slot &s : slots) {
             if (s.ready) (void)hipEventSynchronize(s.ready);
             if (s.used) (void)hipEventSynchronize(s.used);
             if (s.ready) (void)hipEventDestroy(s.ready);
             if (s.used) (void)hipEventDestroy(s.used);
+            s.ready = s.used = nullptr;
         }
+        slots.clear();
+    }
+
+    // Timeline::clear (src/timeline.rs:157-160) exists for when the scene
+    // underneath has changed, so it also forgets the grid size the frames
+    // were bound to and frees their buffers: the next keyframe may come from
+    // a sim of another size.
+    void clear() {
+        ring.clear();
+        drop_slots();
+        cells = 0;
     }
 
     void bind(const dp_sim *sim) {
@@ -185,7 +201,7 @@ int dp_timeline_seek(dp_timeline *t, dp_sim *sim, uint64_t target, int *kind, ui
 int dp_timeline_clear(dp_timeline *t) {
     return guarded([&] {
         if (!t) fail(DP_ERR_INVALID, "null timeline");
-        t->ring.clear();
+        t->clear();
     });
 }
 

@@ -30,6 +30,7 @@ BUFFER_MAGNETIC = 1
 BUFFER_INTENSITY = 2
 BUFFER_MATERIAL_INDEX = 3
 BUFFER_INTENSITY_F32 = 4
+BUFFER_LOOKUP = 5
 
 
 @dataclass
@@ -238,6 +239,11 @@ class Simulation:
         out = c_void_p()
         check(lib().dp_sim_device_ptr(self._handle, which, byref(out)))
         return int(out.value or 0)
+
+    def lookup_samples(self) -> int:
+        """lookup_samples (src/gpu.rs:349-351): samples per axis in the
+        renderer's world-to-cell table (device_ptr(BUFFER_LOOKUP))."""
+        return int(lib().dp_sim_lookup_samples(self._handle))
 
     # -- Steppable (src/timeline.rs:63-72) ---------------------------------
     def snapshot(self) -> Snapshot:

@@ -301,6 +301,24 @@ class Grid:
     def primary(self, axis: Axis) -> np.ndarray:
         return self._resolve()[1][int(axis)]
 
+    def cell_lookup(self, samples: int) -> np.ndarray:
+        """Grid::cell_lookup (src/grid.rs:635-646): 3 × samples fractional
+        cell coordinates, x then y then z, as the engine uploads them."""
+        from ._lib import check, lib
+
+        out = np.zeros(3 * int(samples), dtype=np.float32)
+        check(lib().dp_grid_cell_lookup(ctypes.byref(self.spec()), int(samples),
+                                        out.ctypes.data_as(ctypes.POINTER(c_float)), out.size))
+        return out
+
+    def lookup_samples(self) -> int:
+        """The sample count gpu::Simulation::new picks (src/gpu.rs:225-227)."""
+        from ._lib import check, lib
+
+        out = c_uint32()
+        check(lib().dp_grid_lookup_samples(ctypes.byref(self.spec()), ctypes.byref(out)))
+        return int(out.value)
+
     def dual(self, axis: Axis) -> np.ndarray:
         w = self.primary(axis)
         prev = np.concatenate([w[:1], w[:-1]])

@@ -180,6 +180,16 @@ int dp_grid_resolve(const dp_grid_spec *grid, uint32_t extent_out[3],
                     float *widths_out, size_t widths_capacity,
                     float *time_step_out);
 
+/* Grid::cell_lookup (src/grid.rs:635-646): the renderer's world-to-cell
+ * table. Per axis x, then y, then z, `samples` fractional cell coordinates
+ * (Spacing::coordinate, src/grid.rs:269-279) of the positions
+ * (s/(samples−1) − ½)·length, s = 0..samples−1, all in f32. `out` holds
+ * 3·samples floats; samples >= 2. */
+int dp_grid_cell_lookup(const dp_grid_spec *grid, uint32_t samples, float *out, size_t capacity);
+/* The lookup sample count gpu::Simulation::new picks (src/gpu.rs:225-227):
+ * 4 × the longest axis's cell count, clamped to 256..8192. */
+int dp_grid_lookup_samples(const dp_grid_spec *grid, uint32_t *samples);
+
 /* Scene::validate (src/scene.rs:254-369) for the recoverable problems;
  * DP_OK or DP_ERR_INVALID with the reason in dp_last_error(). */
 int dp_scene_validate(const dp_scene *scene);
@@ -303,9 +313,14 @@ int dp_sim_energy(dp_sim *sim, double *electric, double *magnetic);
  * DP_BUFFER_INTENSITY_F32 rounds the sum to f32 into a buffer of its own,
  * ordered on the sim's stream at this call: the values dp_sim_read_intensity
  * returns, as of this call. */
+/* DP_BUFFER_LOOKUP is lookup_buffer (src/gpu.rs:345-347): the grid's
+ * world-to-cell table, 3·dp_sim_lookup_samples floats as dp_grid_cell_lookup
+ * gives them, uploaded once at dp_sim_create and valid until dp_sim_destroy. */
 enum { DP_BUFFER_ELECTRIC = 0, DP_BUFFER_MAGNETIC = 1, DP_BUFFER_INTENSITY = 2,
-       DP_BUFFER_MATERIAL_INDEX = 3, DP_BUFFER_INTENSITY_F32 = 4 };
+       DP_BUFFER_MATERIAL_INDEX = 3, DP_BUFFER_INTENSITY_F32 = 4, DP_BUFFER_LOOKUP = 5 };
 int dp_sim_device_ptr(dp_sim *sim, int which, void **out);
+/* lookup_samples (src/gpu.rs:349-351): samples per axis in DP_BUFFER_LOOKUP. */
+uint32_t dp_sim_lookup_samples(const dp_sim *sim);
 
 /* Kernel path for the forward step. FUSED (default): one z-marching pass
  * per step doing H, H-injections, E, E-injections and intensity.
@@ -396,7 +411,10 @@ int dp_timeline_observe(dp_timeline *timeline, dp_sim *sim);
  * pointer may be null). */
 int dp_timeline_seek(dp_timeline *timeline, dp_sim *sim, uint64_t target, int *kind, uint64_t *replayed,
                      uint64_t *at);
-/* Timeline::clear. */
+/* Timeline::clear (src/timeline.rs:157-160), for when the scene underneath
+ * has changed: drops every keyframe, frees their device memory and forgets
+ * the grid size, so the next keyframe may come from a sim of another size.
+ * Waits for pending copies into or out of the ring. */
 int dp_timeline_clear(dp_timeline *timeline);
 /* keyframe_count, earliest (0 when there is none) and bytes (:150-165). */
 int dp_timeline_info(const dp_timeline *timeline, uint32_t *keyframes, uint64_t *earliest, uint64_t *bytes);

@@ -42,6 +42,7 @@ def lib() -> ctypes.CDLL:
             "or_grid_packed_geometry": (c_int, [POINTER(DpGridSpec), POINTER(c_float), c_size_t]),
             "or_grid_coordinate": (c_float, [POINTER(DpGridSpec), c_int, c_float]),
             "or_grid_position": (c_float, [POINTER(DpGridSpec), c_int, c_float]),
+            "or_grid_cell_lookup": (c_int, [POINTER(DpGridSpec), c_uint32, POINTER(c_float), c_size_t]),
             "or_grid_cell_containing": (c_int, [POINTER(DpGridSpec), POINTER(c_float), POINTER(c_uint32)]),
             "or_grid_frequency_for_resolution": (c_float, [POINTER(DpGridSpec), c_float]),
             "or_grid_worst_ratio": (c_float, [POINTER(DpGridSpec)]),
@@ -134,6 +135,14 @@ def coordinate(grid, axis: int, position: float) -> float:
 
 def position(grid, axis: int, coordinate_value: float) -> float:
     return float(lib().or_grid_position(byref(grid.spec()), int(axis), float(coordinate_value)))
+
+
+def cell_lookup(grid, samples: int) -> np.ndarray:
+    """Grid::cell_lookup (src/grid.rs:635-646)."""
+    out = np.zeros(3 * int(samples), dtype=np.float32)
+    _check(lib().or_grid_cell_lookup(byref(grid.spec()), int(samples), out.ctypes.data_as(POINTER(c_float)),
+                                     out.size))
+    return out
 
 
 def cell_containing(grid, at) -> tuple:

@@ -396,6 +396,25 @@ float or_grid_position(const dp_grid_spec *spec, int axis, float coordinate) {
     return v;
 }
 
+/* Grid::cell_lookup, src/grid.rs:635-646: per axis, `samples` fractional
+ * cell coordinates of positions from wall to wall, f32 throughout. */
+int or_grid_cell_lookup(const dp_grid_spec *spec, uint32_t samples, float *out, size_t capacity) {
+    if (samples < 2) return fail("a lookup needs at least two samples");
+    if (capacity < (size_t)3 * samples) return fail("lookup buffer too small");
+    grid_t g;
+    if (grid_build(&g, spec) != 0) return -1;
+    size_t at = 0;
+    for (int a = 0; a < 3; ++a) {
+        float length = spacing_length(&g.spacing[a]);
+        for (uint32_t i = 0; i < samples; ++i) {
+            float fraction = (float)i / (float)(samples - 1);
+            out[at++] = spacing_coordinate(&g.spacing[a], (fraction - 0.5f) * length);
+        }
+    }
+    grid_free(&g);
+    return 0;
+}
+
 int or_grid_cell_containing(const dp_grid_spec *spec, const float position[3], uint32_t out[3]) {
     grid_t g;
     if (grid_build(&g, spec) != 0) return -1;

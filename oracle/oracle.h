@@ -39,6 +39,7 @@ int or_grid_packed_geometry(const dp_grid_spec *grid, float *out,
 float or_grid_coordinate(const dp_grid_spec *grid, int axis, float position);
 float or_grid_position(const dp_grid_spec *grid, int axis, float coordinate);
 /* Grid::cell_containing (src/grid.rs:710-714). */
+int or_grid_cell_lookup(const dp_grid_spec *spec, uint32_t samples, float *out, size_t capacity);
 int or_grid_cell_containing(const dp_grid_spec *grid, const float position[3],
                             uint32_t out[3]);
 /* numerical_phase_velocity (src/grid.rs:966-1012). */

@@ -76,7 +76,7 @@ def test_python_constants_match_the_header():
         "DP_TILE_BUFFER": gpu.TILE_BUFFER, "DP_TILE_TWO_STEP": gpu.TILE_TWO_STEP,
         "DP_BUFFER_ELECTRIC": gpu.BUFFER_ELECTRIC, "DP_BUFFER_MAGNETIC": gpu.BUFFER_MAGNETIC,
         "DP_BUFFER_INTENSITY": gpu.BUFFER_INTENSITY, "DP_BUFFER_MATERIAL_INDEX": gpu.BUFFER_MATERIAL_INDEX,
-        "DP_BUFFER_INTENSITY_F32": gpu.BUFFER_INTENSITY_F32,
+        "DP_BUFFER_INTENSITY_F32": gpu.BUFFER_INTENSITY_F32, "DP_BUFFER_LOOKUP": gpu.BUFFER_LOOKUP,
         "DP_BATCH_WAVE_PER_TILE": batch.WAVE_PER_TILE,
     }
     for name, value in pairs.items():
