@@ -45,14 +45,20 @@ def test_errors_come_back_as_status_codes():
 
 
 def test_engine_creation_without_a_device_fails_loudly():
-    import torch
-
-    if torch.cuda.is_available():
-        pytest.skip("a device is present")
+    # The engine answers whether a device exists: dp_engine_create opens one
+    # or fails with DP_ERR_DEVICE. A device index past the last one fails
+    # the same way on any machine.
     from diaphane_amd.gpu import Context
 
-    with pytest.raises(_lib.DiaphaneError):
-        Context(0)
+    with pytest.raises(_lib.DiaphaneError) as caught:
+        Context(4096)
+    assert caught.value.code == _lib.DP_ERR_DEVICE
+    try:
+        context = Context(0)
+    except _lib.DiaphaneError as error:
+        assert error.code == _lib.DP_ERR_DEVICE, str(error)
+        return
+    assert context.arch().startswith("gfx")
 
 
 def header_enums():

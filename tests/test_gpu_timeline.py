@@ -160,3 +160,50 @@ def test_the_ring_refuses_another_grid_and_batch_members(context):
     with pytest.raises(_lib.DiaphaneError, match="batch"):
         timeline.observe(member)
     batch.close()
+
+
+def test_a_snapshot_restores_the_exact_state(context):
+    # src/timeline.rs:232-245, through Steppable::snapshot/restore on the device
+    simulation = Simulation(context, scene())
+    simulation.advance_by(60)
+    snapshot = simulation.snapshot()
+    expected = simulation.read_electric()
+    simulation.advance_by(40)
+    assert not bits_equal(simulation.read_electric(), expected)
+    simulation.restore(snapshot)
+    assert simulation.step_count() == 60
+    assert bits_equal(simulation.read_electric(), expected)
+    assert bits_equal(expected, oracle_after(60).electric_all)
+    assert snapshot.bytes() == 24 * Extent.cube(24).total()  # src/timeline.rs:385-390
+    simulation.close()
+
+
+def test_scrubbing_a_lossless_scene_agrees_with_stepping_backwards(context):
+    # src/timeline.rs:392-420: replay from a device keyframe and the exact
+    # inverse update land on the same state; each route is also the oracle's.
+    from diaphane_amd import Boundary
+
+    lossless = Scene.cavity(Extent.cube(24)).with_boundary(Boundary.PEC)
+    scrubbed = Simulation(context, lossless)
+    timeline = DeviceTimeline(20, 8)
+    for _ in range(100):
+        scrubbed.advance_by(1)
+        timeline.observe(scrubbed)
+    reversed_ = Simulation(context, lossless)
+    reversed_.advance_by(100)
+    reversed_.reverse_by(30)
+    timeline.seek(scrubbed, 70)
+    assert scrubbed.step_count() == reversed_.step_count() == 70
+    a, b = scrubbed.read_electric(), reversed_.read_electric()
+    peak = float(np.abs(a).max())
+    assert float(np.abs(a - b).max()) < 1e-4 * peak
+    forward = oracle.Simulation(lossless)
+    forward.advance_by(70)
+    assert bits_equal(a, forward.electric_all)
+    backward = oracle.Simulation(lossless)
+    backward.advance_by(100)
+    backward.reverse_by(30)
+    assert bits_equal(b, backward.electric_all)
+    for sim in (scrubbed, reversed_):
+        sim.close()
+    timeline.close()
