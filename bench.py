@@ -8,7 +8,11 @@ z-dipole, 1000 steps with |E|² accumulated (f64) over the last 500. Fields and
 materials are resident in HBM before the timed region starts.
 
 One step = one full leapfrog step of the whole grid (H, injections, E,
-injections, intensity). Prints one JSON line.
+injections, intensity). Prints one JSON line. The default line also carries
+``batched`` and ``served`` objects: the two workloads below, each against the
+same work run one solve at a time on the same GPU (``--no-batched``,
+``--no-served`` leave them out), and ``cpu_baseline`` times the C oracle on
+all the run's host threads and on one.
 
 ``--workload batched_uneven_256``: 256 independent uneven solves (sides
 log-uniform in [48, 160] per axis, PEC or absorbing, seeded cavity modes),
@@ -37,22 +41,12 @@ BYTES_PER_CELL = 49            # 6 f32 fields read+written + 1 u8 material index
 BYTES_PER_CELL_INTENSITY = 65  # + f64 intensity read-modify-write
 
 
-def cpu_baseline(workload_name: str, seconds: float = 10.0) -> dict:
-    """Times the C oracle (a restatement of the reference's cpu.rs) on the
-    host's cores over a bounded sample of the same workload."""
-    import numpy as np  # noqa: F401
-
+def time_oracle(scene, threads: int, seconds: float) -> tuple:
+    """Steps the C oracle on `threads` host threads until `seconds` have
+    passed (at least 3 steps, after 2 warm-up): (cell-updates/s, steps, s)."""
     import oracle
-    from diaphane_amd import workloads
 
-    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or min(os.cpu_count() or 1, 16)
-    if workload_name == "layered_768":
-        sample = workloads.layered_768(nz=64)
-        label = "768x768x64 slab of the layered_768 stack"
-    else:
-        sample = workloads.WORKLOADS[workload_name]()
-        label = f"{workload_name} (full grid)"
-    sim = oracle.Simulation(sample.scene, threads=threads)
+    sim = oracle.Simulation(scene, threads=threads)
     sim.advance_by(2)  # warm-up
     steps = 0
     start = time.perf_counter()
@@ -62,13 +56,37 @@ def cpu_baseline(workload_name: str, seconds: float = 10.0) -> dict:
         elapsed = time.perf_counter() - start
         if elapsed >= seconds and steps >= 3:
             break
-    rate = sim.cells * steps / elapsed
+    return sim.cells * steps / elapsed, steps, elapsed
+
+
+def cpu_baseline(workload_name: str, seconds: float = 10.0) -> dict:
+    """Times the C oracle (a restatement of the reference's cpu.rs) on the
+    host's cores over a bounded sample of the same workload: once on every
+    core the run may use (OpenMP over z, bit-identical to one thread), and
+    once on a single thread, the reference's own timing discipline."""
+    from diaphane_amd import workloads
+
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or min(os.cpu_count() or 1, 16)
+    if workload_name == "layered_768":
+        sample = workloads.layered_768(nz=64)
+        label = "768x768x64 slab of the layered_768 stack"
+    else:
+        sample = workloads.WORKLOADS[workload_name]()
+        label = f"{workload_name} (full grid)"
+    rate, steps, elapsed = time_oracle(sample.scene, threads, seconds)
+    one, one_steps, one_elapsed = time_oracle(sample.scene, 1, seconds)
     return {
         "value": rate,
         "unit": "cell-updates/s",
         "cores": threads,
         "kind": "port",
         "sample": f"{label}: {steps} steps after 2 warm-up, {elapsed:.1f} s, {threads} threads",
+        "single_thread": {
+            "value": one,
+            "unit": "cell-updates/s",
+            "cores": 1,
+            "sample": f"{label}: {one_steps} steps after 2 warm-up, {one_elapsed:.1f} s, 1 thread",
+        },
     }
 
 
@@ -109,18 +127,18 @@ def batch_cpu_baseline(instances, seconds: float) -> dict:
     }
 
 
-def bench_batched(args) -> int:
-    """Aggregate cell-updates/s of the batched form against the same solves
-    run one at a time on the same GPU."""
+def run_batched(context, steps: int, warmup: int, tile=None) -> dict:
+    """The 256 uneven solves run one at a time, then as one batch from the
+    same initial fields; eight members are compared bit for bit between the
+    two runs. Returns the measurements (tile: (rows, depth, flags) or None
+    for the autotuner)."""
     import numpy as np
 
     from diaphane_amd import workloads
     from diaphane_amd.batch import Batch
-    from diaphane_amd.gpu import Simulation, headless_context
+    from diaphane_amd.gpu import Simulation
 
     instances = workloads.batched_uneven_256()
-    steps = args.steps if args.steps is not None else 1000
-    context = headless_context()
     log(f"creating {len(instances)} sims")
     sims = []
     initial = []
@@ -137,8 +155,8 @@ def bench_batched(args) -> int:
 
     # One at a time: each solve runs its steps alone, then the next.
     log("one at a time")
-    if args.warmup:
-        sims[0].advance_by(args.warmup)
+    if warmup:
+        sims[0].advance_by(warmup)
         sims[0].wait()
         sims[0].write_fields(*initial[0], 0)
     seq_kernel_ms = 0.0
@@ -159,8 +177,8 @@ def bench_batched(args) -> int:
     batch = Batch(context)
     for sim in sims:
         batch.add(sim, steps)
-    if args.rows:
-        batch.set_tile(args.rows, args.depth, args.tile_flags)
+    if tile:
+        batch.set_tile(*tile)
         tuned = dict(batch.tiling(), ms_per_step=None)
     else:
         rows, depth, ms = batch.autotune(3)
@@ -179,11 +197,73 @@ def bench_batched(args) -> int:
         np.array_equal(sims[k].read_electric().view(np.uint32), solo_fields[k][0].view(np.uint32))
         and np.array_equal(sims[k].read_magnetic().view(np.uint32), solo_fields[k][1].view(np.uint32))
         for k in check)
+    launches = batch.launch_count()
+    batch.close()
+    for sim in sims:
+        sim.close()
 
     updates = total_cells * steps
-    value = updates / batch_wall
-    seq_value = updates / seq_wall
     achieved = BYTES_PER_CELL * updates / (batch_ms / 1e3) / 1e9
+    log(f"  batched {updates / batch_wall / 1e9:.1f} G/s, one at a time {updates / seq_wall / 1e9:.1f} G/s")
+    return {
+        "instances": instances,
+        "value": updates / batch_wall,
+        "wall_s": batch_wall,
+        "steps": steps,
+        "cells": total_cells,
+        "cells_min_max": [min(cells), max(cells)],
+        "pec_instances": sum(1 for i in instances if i.scene.boundary.kind == "pec"),
+        "sequential": {
+            "value": updates / seq_wall,
+            "unit": "cell-updates/s",
+            "wall_s": seq_wall,
+            "kernel_s": seq_kernel_ms / 1e3,
+            "note": "the same solves from the same initial fields, dp_sim_advance one solve after another",
+        },
+        "speedup_vs_sequential": seq_wall / batch_wall,
+        "members_bit_identical_to_solo": identical,
+        "members_checked": len(check),
+        "tiling": tuned,
+        "kernel": "k_fused_pair_batch" if tuned.get("two_step") else
+                  ("k_fused_batch_direct" if tuned.get("wave_per_tile") else "k_fused_batch"),
+        "launches": launches,
+        "kernel_ms": batch_ms,
+        "achieved_gbs": achieved,
+    }
+
+
+def batched_summary(run: dict) -> dict:
+    """The batched form's sub-object of the default benchmark line."""
+    return {
+        "workload": "batched_uneven_256",
+        "metric": "aggregate Yee cell-updates/s, batched vs one at a time",
+        "unit": "cell-updates/s",
+        "value": run["value"],
+        "sequential_value": run["sequential"]["value"],
+        "speedup_vs_sequential": run["speedup_vs_sequential"],
+        "members_bit_identical_to_solo": run["members_bit_identical_to_solo"],
+        "members_checked": run["members_checked"],
+        "instances": len(run["instances"]),
+        "cells": run["cells"],
+        "steps": run["steps"],
+        "wall_s": run["wall_s"],
+        "sequential_wall_s": run["sequential"]["wall_s"],
+        "kernel": run["kernel"],
+        "tiling": run["tiling"],
+        "achieved_gbs": run["achieved_gbs"],
+        "frac_of_hbm_peak": run["achieved_gbs"] / HBM_PEAK_GBS,
+    }
+
+
+def bench_batched(args) -> int:
+    """Aggregate cell-updates/s of the batched form against the same solves
+    run one at a time on the same GPU."""
+    from diaphane_amd.gpu import headless_context
+
+    steps = args.steps if args.steps is not None else 1000
+    tile = (args.rows, args.depth, args.tile_flags) if args.rows else None
+    run = run_batched(headless_context(), steps, args.warmup, tile)
+    value = run["value"]
     result = {
         "metric": "aggregate Yee cell-updates/s, batched vs one at a time",
         "value": value,
@@ -191,7 +271,7 @@ def bench_batched(args) -> int:
         "n_gpus": 1,
         "steps": steps,
         "warmup": args.warmup,
-        "ms_per_step": batch_wall * 1e3 / steps,
+        "ms_per_step": run["wall_s"] * 1e3 / steps,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -199,39 +279,31 @@ def bench_batched(args) -> int:
         "data": "synthetic",
         "config": {
             "workload": "batched_uneven_256",
-            "instances": len(sims),
-            "cells": total_cells,
-            "cells_min_max": [min(cells), max(cells)],
-            "pec_instances": sum(1 for i in instances if i.scene.boundary.kind == "pec"),
+            "instances": len(run["instances"]),
+            "cells": run["cells"],
+            "cells_min_max": run["cells_min_max"],
+            "pec_instances": run["pec_instances"],
         },
-        "sequential": {
-            "value": seq_value,
-            "unit": "cell-updates/s",
-            "wall_s": seq_wall,
-            "kernel_s": seq_kernel_ms / 1e3,
-            "note": "the same solves from the same initial fields, dp_sim_advance one solve after another",
-        },
-        "speedup_vs_sequential": value / seq_value,
-        "members_bit_identical_to_solo": identical,
-        "members_checked": len(check),
-        "autotune": tuned,
+        "sequential": run["sequential"],
+        "speedup_vs_sequential": run["speedup_vs_sequential"],
+        "members_bit_identical_to_solo": run["members_bit_identical_to_solo"],
+        "members_checked": run["members_checked"],
+        "autotune": run["tiling"],
         "roofline": {
             "bound": "hbm",
-            "achieved": achieved,
+            "achieved": run["achieved_gbs"],
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS,
+            "frac": run["achieved_gbs"] / HBM_PEAK_GBS,
             "traffic": None,
-            "kernel": "k_fused_pair_batch" if tuned.get("two_step") else
-                      ("k_fused_batch_direct" if tuned.get("wave_per_tile") else "k_fused_batch"),
+            "kernel": run["kernel"],
             "bytes_per_cell": BYTES_PER_CELL,
-            "launches": batch.launch_count(),
-            "avg_launch_ms": batch_ms / max(batch.launch_count(), 1),
+            "launches": run["launches"],
+            "avg_launch_ms": run["kernel_ms"] / max(run["launches"], 1),
         },
     }
-    batch.close()
     if not args.no_cpu:
-        cpu = batch_cpu_baseline(instances, args.cpu_seconds)
+        cpu = batch_cpu_baseline(run["instances"], args.cpu_seconds)
         result["cpu_baseline"] = cpu
         result["speedup_vs_cpu"] = value / cpu["value"]
     print(json.dumps(result), flush=True)
@@ -298,29 +370,66 @@ def serve_stream(context, stream, one_at_a_time: bool, chunk_steps: int, rows: i
     }
 
 
-def bench_served(args) -> int:
+def run_served(context, requests: int, rate: float, chunk_steps: int, rows: int, chunk_cells: int) -> dict:
+    """The Poisson request stream served with continuous batching, then the
+    same stream (same arrival times) served one request at a time."""
     from diaphane_amd import workloads
-    from diaphane_amd.gpu import headless_context
 
-    stream = workloads.served_stream(args.requests, args.rate)
-    context = headless_context()
+    stream = workloads.served_stream(requests, rate)
     updates = sum(r.scene.grid.extent.total() * r.steps for r in stream)
     log(f"{len(stream)} requests over {stream[-1].arrival_s:.1f} s, {updates / 1e9:.0f} G cell-updates")
     # warm the code paths (kernels load on first launch)
     warm = workloads.served_stream(3, 1000.0, seed=99, lo=32, hi=40, steps_lo=40, steps_hi=40)
-    chunk_cells = (1 << 64) - 1 if args.chunk_cells < 0 else args.chunk_cells
-    serve_stream(context, warm, False, args.chunk_steps, args.rows, 0, chunk_cells)
-    serve_stream(context, warm, True, args.chunk_steps, args.rows, 0)
+    cells_per_chunk = (1 << 64) - 1 if chunk_cells < 0 else chunk_cells
+    serve_stream(context, warm, False, chunk_steps, rows, 0, cells_per_chunk)
+    serve_stream(context, warm, True, chunk_steps, rows, 0)
     log("continuous batching")
-    batched = serve_stream(context, stream, False, args.chunk_steps, args.rows, 0, chunk_cells)
+    batched = serve_stream(context, stream, False, chunk_steps, rows, 0, cells_per_chunk)
     log(f"  {batched['value'] / 1e9:.1f} G/s, p50 {batched['latency_s']['p50']:.2f} s")
     log("one at a time")
-    sequential = serve_stream(context, stream, True, args.chunk_steps, args.rows, 0)
+    sequential = serve_stream(context, stream, True, chunk_steps, rows, 0)
     log(f"  {sequential['value'] / 1e9:.1f} G/s, p50 {sequential['latency_s']['p50']:.2f} s")
-    kinds = {k: sum(1 for r in stream if r.kind == k) for k in workloads.SERVED_KINDS}
+    return {
+        "stream": stream,
+        "cell_updates": updates,
+        "served": batched,
+        "sequential": sequential,
+        "speedup_vs_sequential": batched["value"] / sequential["value"],
+        "kinds": {k: sum(1 for r in stream if r.kind == k) for k in workloads.SERVED_KINDS},
+        "chunk_cells": "every member" if chunk_cells < 0 else (chunk_cells or 64 << 20),
+    }
+
+
+def served_summary(run: dict, rate: float, chunk_steps: int) -> dict:
+    """The served form's sub-object of the default benchmark line."""
+    return {
+        "workload": "served_stream",
+        "metric": "served aggregate Yee cell-updates/s with p50/p99 completion latency",
+        "unit": "cell-updates/s",
+        "value": run["served"]["value"],
+        "latency_s": run["served"]["latency_s"],
+        "sequential_value": run["sequential"]["value"],
+        "sequential_latency_s": run["sequential"]["latency_s"],
+        "speedup_vs_sequential": run["speedup_vs_sequential"],
+        "requests": len(run["stream"]),
+        "rate_per_s": rate,
+        "cell_updates": run["cell_updates"],
+        "kinds": run["kinds"],
+        "hbm_budget_bytes": 200 << 30,
+        "peak_resident_bytes": run["served"]["peak_resident_bytes"],
+        "chunk_steps": chunk_steps,
+        "chunk_cells": run["chunk_cells"],
+    }
+
+
+def bench_served(args) -> int:
+    from diaphane_amd.gpu import headless_context
+
+    run = run_served(headless_context(), args.requests, args.rate, args.chunk_steps, args.rows, args.chunk_cells)
+    stream = run["stream"]
     result = {
         "metric": "served aggregate Yee cell-updates/s with p50/p99 completion latency",
-        "value": batched["value"],
+        "value": run["served"]["value"],
         "unit": "cell-updates/s",
         "n_gpus": 1,
         "steps": int(sum(r.steps for r in stream)),
@@ -335,15 +444,15 @@ def bench_served(args) -> int:
             "workload": "served_stream",
             "requests": len(stream),
             "rate_per_s": args.rate,
-            "cell_updates": updates,
-            "kinds": kinds,
+            "cell_updates": run["cell_updates"],
+            "kinds": run["kinds"],
             "hbm_budget_bytes": 200 << 30,
             "chunk_steps": args.chunk_steps,
-            "chunk_cells": "every member" if args.chunk_cells < 0 else (args.chunk_cells or 64 << 20),
+            "chunk_cells": run["chunk_cells"],
         },
-        "served": batched,
-        "sequential": sequential,
-        "speedup_vs_sequential": batched["value"] / sequential["value"],
+        "served": run["served"],
+        "sequential": run["sequential"],
+        "speedup_vs_sequential": run["speedup_vs_sequential"],
     }
     print(json.dumps(result), flush=True)
     return 0
@@ -361,6 +470,10 @@ def main() -> int:
                         help="tile flags (1 XCD remap, 2 non-temporal stores, 4 buffer addressing, 8 two steps "
                              "per launch; batched: 16 one wave per tile; -1 = default)")
     parser.add_argument("--no-cpu", action="store_true", help="skip the CPU-oracle baseline")
+    parser.add_argument("--no-batched", action="store_true",
+                        help="default workload: skip the batched_uneven_256 comparison")
+    parser.add_argument("--no-served", action="store_true",
+                        help="default workload: skip the served_stream comparison")
     parser.add_argument("--cpu-seconds", type=float, default=10.0)
     parser.add_argument("--requests", type=int, default=200, help="served_stream: number of requests")
     parser.add_argument("--rate", type=float, default=50.0, help="served_stream: arrivals per second")
@@ -549,7 +662,18 @@ def main() -> int:
             },
         },
     }
+    if traffic is not None:
+        result["roofline"]["traffic_source"] = (
+            f"profiles/{os.path.basename(traffic_file)}: rocprofv3 FETCH_SIZE x 2 + WRITE_SIZE per launch of "
+            f"this kernel at this shape, from a separate counter run (not measured in this run)")
     sim.close()
+    if work.name == "layered_768":
+        # The batched and served forms on their own workloads, each against
+        # the same work run one solve at a time on this GPU.
+        if not args.no_batched:
+            result["batched"] = batched_summary(run_batched(context, 1000, args.warmup))
+        if not args.no_served:
+            result["served"] = served_summary(run_served(context, 200, 50.0, 32, 0, 0), 50.0, 32)
     if not args.no_cpu:
         cpu = cpu_baseline(args.workload, args.cpu_seconds)
         result["cpu_baseline"] = cpu
