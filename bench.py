@@ -224,7 +224,8 @@ def run_batched(context, steps: int, warmup: int, tile=None) -> dict:
         "members_bit_identical_to_solo": identical,
         "members_checked": len(check),
         "tiling": tuned,
-        "kernel": "k_fused_pair_batch" if tuned.get("two_step") else
+        "kernel": ("k_fused_stack_batch" if tuned.get("stack_waves") else "k_fused_pair_batch")
+                  if tuned.get("two_step") else
                   ("k_fused_batch_direct" if tuned.get("wave_per_tile") else "k_fused_batch"),
         "launches": launches,
         "kernel_ms": batch_ms,
@@ -367,6 +368,9 @@ def serve_stream(context, stream, one_at_a_time: bool, chunk_steps: int, rows: i
         "peak_resident_bytes": peak_bytes,
         "chunks": stats["chunks"],
         "device_busy_s": stats["busy_s"],
+        "kernel_s": stats["kernel_s"],
+        "kernel_rate": stats["cell_updates"] / stats["kernel_s"] if stats["kernel_s"] else None,
+        "device_utilization": stats["kernel_s"] / (last - first),
     }
 
 
@@ -417,6 +421,9 @@ def served_summary(run: dict, rate: float, chunk_steps: int) -> dict:
         "kinds": run["kinds"],
         "hbm_budget_bytes": 200 << 30,
         "peak_resident_bytes": run["served"]["peak_resident_bytes"],
+        "device_utilization": run["served"]["device_utilization"],
+        "kernel_rate": run["served"]["kernel_rate"],
+        "sequential_kernel_rate": run["sequential"]["kernel_rate"],
         "chunk_steps": chunk_steps,
         "chunk_cells": run["chunk_cells"],
     }
@@ -496,7 +503,7 @@ def main() -> int:
         return bench_served(args)
 
     from diaphane_amd import workloads
-    from diaphane_amd.gpu import TILE_TWO_STEP, Simulation, headless_context
+    from diaphane_amd.gpu import TILE_STACK, TILE_TWO_STEP, Simulation, headless_context
 
     work = workloads.WORKLOADS[args.workload]()
     steps = args.steps if args.steps is not None else work.steps
@@ -596,7 +603,9 @@ def main() -> int:
 
     # The kernel each phase ran: two steps per launch when its tiling says so.
     def kernel_of(flags: int) -> str:
-        return "k_fused_pair" if flags >= 0 and flags & TILE_TWO_STEP else "k_fused_aligned"
+        if flags >= 0 and flags & TILE_TWO_STEP:
+            return "k_fused_stack" if flags & TILE_STACK else "k_fused_pair"
+        return "k_fused_aligned"
 
     plain_flags = tiling.get("flags", -1)
     kernels = {"plain": kernel_of(plain_flags),

@@ -27,7 +27,7 @@ class DpRequestInfo(ctypes.Structure):
 class DpServerStats(ctypes.Structure):
     _fields_ = [("queued", c_uint32), ("preparing", c_uint32), ("ready", c_uint32), ("running", c_uint32),
                 ("used_bytes", c_uint64), ("budget_bytes", c_uint64), ("chunks", c_uint64), ("busy_s", c_double),
-                ("uptime_s", c_double)]
+                ("uptime_s", c_double), ("kernel_s", c_double), ("cell_updates", c_uint64)]
 
 LIBRARY_PATH = Path(__file__).resolve().parent / "libdiaphane_amd.so"
 

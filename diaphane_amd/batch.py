@@ -80,6 +80,7 @@ class Batch:
         return {"rows": rows.value, "depth": depth.value, "wave_per_tile": bool(flags.value & WAVE_PER_TILE),
                 "buffer": bool(flags.value & 4), "nontemporal": bool(flags.value & 2),
                 "two_step": bool(flags.value & 8),
+                "stack_waves": ((flags.value >> 16) & 0xF or 4) if flags.value & 32 else 0,
                 "xcd_group": (((flags.value >> 8) & 0xFF) or 8) if flags.value & 1 else 0}
 
     def __len__(self) -> int:

@@ -36,6 +36,7 @@ HOST = ["-x", "c++", "-D__HIP_PLATFORM_AMD__", f"-I{ROCM}/include"]
 SOURCES = [
     ("kernels.hip", DEVICE),
     ("kernels_pair.hip", DEVICE),
+    ("kernels_stack.hip", DEVICE),
     ("scene.cpp", HOST),
     ("scene_ron.cpp", HOST),
     ("timeline.cpp", HOST),

@@ -14,6 +14,7 @@
 // alternate so the host prepares chunk n+1 while the device runs chunk n.
 #include "engine.hpp"
 
+#include <algorithm>
 #include <limits>
 
 namespace {
@@ -116,8 +117,11 @@ struct dp_batch {
         dirty = true;
     }
 
-    // Two steps per launch use one wave per tile.
+    // Two steps per launch use one wave per tile, or with DP_TILE_STACK one
+    // workgroup of stacked waves per tile.
     bool two_step() const { return (flags & DP_TILE_TWO_STEP) != 0; }
+    bool stack() const { return two_step() && (flags & DP_TILE_STACK) != 0; }
+    int waves() const { return dp_sim::stack_waves(flags); }
     bool direct() const { return (flags & DP_BATCH_WAVE_PER_TILE) != 0 || two_step(); }
     // Workgroups per XCD run for the two-step kernel (DP_TILE_XCD_REMAP,
     // group in bits 8..15, 0 meaning 8); 0 without the flag.
@@ -145,7 +149,7 @@ struct dp_batch {
     void sync() { check(hipStreamSynchronize(stream), "hipStreamSynchronize"); }
 
     uint32_t owned_x() const { return two_step() ? kPairOwnedX : kAlignedX; }
-    int owned_rows() const { return rows - (two_step() ? 4 : 2); }
+    int owned_rows() const { return stack() ? waves() * rows - 4 : rows - (two_step() ? 4 : 2); }
 
     void tiles_of(const dp_sim *sim, InstanceDesc &d) const {
         const uint32_t nx = sim->grid.extent[0], ny = sim->grid.extent[1], nz = sim->grid.extent[2];
@@ -239,7 +243,11 @@ struct dp_batch {
         if (!trial_injections.ptr) trial_injections.allocate(sizeof(InjDesc));
         const bool nt = (flags & DP_TILE_NONTEMPORAL) != 0;
         auto launch = [&] {
-            if (two_step())
+            if (stack())
+                launch_fused_stack_batch(instance_table.as<InstanceDesc>(), trial_slots.as<StepSlot>(),
+                                         owners.as<uint32_t>(), n_tiles, trial_injections.as<InjDesc>(), rows, waves(),
+                                         buffer(), xcd_group(), stream);
+            else if (two_step())
                 launch_fused_pair_batch(instance_table.as<InstanceDesc>(), trial_slots.as<StepSlot>(),
                                         owners.as<uint32_t>(), n_tiles, trial_injections.as<InjDesc>(), rows, nt,
                                         buffer(), xcd_group(), stream);
@@ -264,6 +272,7 @@ struct dp_batch {
     void autotune(uint32_t trials, int *rows_out, int *depth_out, float *ms_out) {
         static const int row_choices[] = {4, 6, 8};
         static const int pair_row_choices[] = {6, 8, 10};
+        static const int stack_row_choices[] = {4, 6, 6};  // (6 twice: the sweep takes three)
         static const uint32_t depth_choices[] = {8, 16, 32, 64};
         // One step per launch keeps non-temporal stores (measured faster);
         // two steps per launch try both store kinds.
@@ -275,28 +284,54 @@ struct dp_batch {
             DP_TILE_TWO_STEP,
             DP_TILE_TWO_STEP | DP_TILE_BUFFER,
             DP_TILE_TWO_STEP | DP_TILE_NONTEMPORAL,
-            DP_TILE_TWO_STEP | DP_TILE_NONTEMPORAL | DP_TILE_BUFFER};
-        const int tuned_bits = DP_BATCH_WAVE_PER_TILE | DP_TILE_BUFFER | DP_TILE_TWO_STEP | DP_TILE_NONTEMPORAL;
+            DP_TILE_TWO_STEP | DP_TILE_NONTEMPORAL | DP_TILE_BUFFER,
+            DP_TILE_TWO_STEP | DP_TILE_STACK | DP_TILE_BUFFER | (2 << 16),
+            DP_TILE_TWO_STEP | DP_TILE_STACK | DP_TILE_BUFFER | (4 << 16)};
+        const int tuned_bits =
+            DP_BATCH_WAVE_PER_TILE | DP_TILE_BUFFER | DP_TILE_TWO_STEP | DP_TILE_NONTEMPORAL | DP_TILE_STACK | 0xF0000;
         int best_rows = rows, best_grid = flags & tuned_bits;
         uint32_t best_depth = depth;
         float best = std::numeric_limits<float>::infinity();
+        struct Candidate {
+            int grid, rows;
+            uint32_t depth;
+            float ms;
+        };
+        auto time_candidate = [&](const Candidate &c, uint32_t launches) {
+            flags = (flags & ~tuned_bits) | c.grid;
+            rows = c.rows;
+            depth = c.depth;
+            dirty = true;
+            return time_tiling(launches);
+        };
         if (!members.empty()) {
+            std::vector<Candidate> sweep;
             for (int g : grid_choices)
                 for (int k = 0; k < 3; ++k)
                     for (uint32_t d : depth_choices) {
-                        const int r = (g & DP_TILE_TWO_STEP) ? pair_row_choices[k] : row_choices[k];
-                        flags = (flags & ~tuned_bits) | g;
-                        rows = r;
-                        depth = d;
-                        dirty = true;
-                        const float ms = time_tiling(trials);
-                        if (ms < best) {
-                            best = ms;
-                            best_rows = r;
-                            best_depth = d;
-                            best_grid = g;
-                        }
+                        if ((g & DP_TILE_STACK) && k == 2) continue;  // stacked tiles: rows 4 and 6
+                        const int r = (g & DP_TILE_STACK)      ? stack_row_choices[k]
+                                      : (g & DP_TILE_TWO_STEP) ? pair_row_choices[k]
+                                                               : row_choices[k];
+                        Candidate c{g, r, d, 0.0f};
+                        c.ms = time_candidate(c, trials);
+                        sweep.push_back(c);
                     }
+            // The sweep's four fastest, timed again with three times the
+            // launches in two alternating rounds: neighbouring shapes differ
+            // by less than one short trial's noise.
+            std::sort(sweep.begin(), sweep.end(), [](const Candidate &x, const Candidate &y) { return x.ms < y.ms; });
+            sweep.resize(std::min<size_t>(sweep.size(), 4));
+            for (Candidate &c : sweep) c.ms = std::numeric_limits<float>::infinity();
+            for (int round = 0; round < 2; ++round)
+                for (Candidate &c : sweep) c.ms = std::min(c.ms, time_candidate(c, 3 * trials));
+            for (const Candidate &c : sweep)
+                if (c.ms < best) {
+                    best = c.ms;
+                    best_rows = c.rows;
+                    best_depth = c.depth;
+                    best_grid = c.grid;
+                }
         }
         flags = (flags & ~tuned_bits) | best_grid;
         rows = best_rows;
@@ -316,7 +351,11 @@ struct dp_batch {
         const bool nt = (flags & DP_TILE_NONTEMPORAL) != 0;
         auto launch_all = [&] {
             for (uint32_t s = 0; s < steps; ++s) {
-                if (two_step())
+                if (stack())
+                    launch_fused_stack_batch(instance_table.as<InstanceDesc>(), slots + size_t(s) * members.size(),
+                                             owners.as<uint32_t>(), n_tiles, inj, rows, waves(), buffer(),
+                                             xcd_group(), stream);
+                else if (two_step())
                     launch_fused_pair_batch(instance_table.as<InstanceDesc>(), slots + size_t(s) * members.size(),
                                             owners.as<uint32_t>(), n_tiles, inj, rows, nt, buffer(), xcd_group(),
                                             stream);
@@ -542,14 +581,19 @@ int dp_batch_set_tile(dp_batch *batch, int rows, int depth, int flags) {
         if (depth < 0) fail(DP_ERR_INVALID, "depth must be non-negative");
         const int new_flags =
             flags >= 0 ? flags & (DP_TILE_NONTEMPORAL | DP_TILE_BUFFER | DP_BATCH_WAVE_PER_TILE | DP_TILE_TWO_STEP |
-                                  DP_TILE_XCD_REMAP | 0xFF00)
+                                  DP_TILE_XCD_REMAP | DP_TILE_STACK | 0xFF00 | 0xF0000)
                        : batch->flags;
         const bool two = (new_flags & DP_TILE_TWO_STEP) != 0;
-        if (rows != 0 && (two ? rows != 6 && rows != 8 && rows != 10 : rows != 4 && rows != 6 && rows != 8))
-            fail(DP_ERR_INVALID, two ? "rows must be 6, 8 or 10 for two steps per launch" : "rows must be 4, 6 or 8");
+        const bool stacked = two && (new_flags & DP_TILE_STACK) != 0;
+        if (rows != 0 && (stacked ? rows != 4 && rows != 6
+                          : two   ? rows != 6 && rows != 8 && rows != 10
+                                  : rows != 4 && rows != 6 && rows != 8))
+            fail(DP_ERR_INVALID, stacked ? "rows must be 4 or 6 for stacked waves"
+                                 : two   ? "rows must be 6, 8 or 10 for two steps per launch"
+                                         : "rows must be 4, 6 or 8");
         batch->sync();
         batch->flags = new_flags;
-        batch->rows = rows ? rows : (two ? 6 : 4);
+        batch->rows = rows ? rows : (stacked ? 4 : two ? 6 : 4);
         batch->depth = static_cast<uint32_t>(depth);
         batch->dirty = true;
     });

@@ -231,8 +231,8 @@ int dp_sim_create(dp_engine *engine, const dp_scene *scene, dp_sim **out) {
         // 2% with the intensity sum).
         if (sim->cells >= dp_sim::kTwoStepDefaultCells &&
             buffer_addressing_fits(sim->cells, g.extent[0], g.extent[1])) {
-            sim->tiling[0] = {10, 0, dp_sim::kTwoStepTileFlags | DP_TILE_BUFFER};
-            sim->tiling[1] = {10, 0, dp_sim::kTwoStepTileFlags | DP_TILE_BUFFER};
+            sim->tiling[0] = {4, 0, dp_sim::kStackTileFlags | DP_TILE_BUFFER};
+            sim->tiling[1] = {4, 0, dp_sim::kStackTileFlags | DP_TILE_BUFFER};
         }
         sim->time_step = g.time_step();
         sim->material_count = scene->material_count;
@@ -529,7 +529,8 @@ int dp_sim_set_tile(dp_sim *sim, int rows, int depth, int flags) {
             t.rows = rows;
             t.depth = static_cast<uint32_t>(depth);
             if (flags >= 0)
-                t.flags = flags & (DP_TILE_XCD_REMAP | DP_TILE_NONTEMPORAL | DP_TILE_BUFFER | DP_TILE_TWO_STEP | 0xFF00);
+                t.flags = flags & (DP_TILE_XCD_REMAP | DP_TILE_NONTEMPORAL | DP_TILE_BUFFER | DP_TILE_TWO_STEP |
+                                   DP_TILE_STACK | 0xFF00 | 0xF0000);
         }
     });
 }
@@ -542,6 +543,8 @@ int dp_sim_autotune(dp_sim *sim, uint32_t trials, int *rows, int *depth, int *fl
         if (trials == 0 || trials > 1000) fail(DP_ERR_INVALID, "trials must be in 1..1000");
         static const int row_choices[] = {4, 6, 8};
         static const int pair_row_choices[] = {6, 8, 10};
+        static const int stack_row_choices[] = {4, 6};
+        static const int stack_wave_choices[] = {2, 4};
         static const uint32_t depth_choices[] = {1, 2, 4, 8, 16, 32, 64};
         const bool buffer_fits = buffer_addressing_fits(sim->cells, sim->grid.extent[0], sim->grid.extent[1]);
         const int flag_choices[] = {DP_TILE_NONTEMPORAL, DP_TILE_NONTEMPORAL | DP_TILE_BUFFER, 0, DP_TILE_BUFFER};
@@ -571,10 +574,10 @@ int dp_sim_autotune(dp_sim *sim, uint32_t trials, int *rows, int *depth, int *fl
             trial.rows = r;
             trial.depth = d;
             trial.flags = f;
-            sim->fill_tiles(a, r, two_step ? kPairOwnedX : kAlignedX, trial, two_step ? 4 : 2);
+            if (!two_step) sim->fill_tiles(a, r, kAlignedX, trial, 2);
             auto launch = [&] {
                 if (two_step)
-                    launch_fused_pair(a, r, (f & DP_TILE_NONTEMPORAL) != 0, (f & DP_TILE_BUFFER) != 0, sim->stream);
+                    sim->launch_pair_args(a, trial, sim->stream);
                 else
                     launch_fused_aligned(a, r, (f & DP_TILE_NONTEMPORAL) != 0, (f & DP_TILE_BUFFER) != 0,
                                          sim->stream);
@@ -606,6 +609,14 @@ int dp_sim_autotune(dp_sim *sim, uint32_t trials, int *rows, int *depth, int *fl
                     const int pf = f | dp_sim::kTwoStepTileFlags;
                     sweep.push_back({r, d, pf, time_shape(r, d, pf, trials)});
                 }
+                // Waves stacked along y (cached stores only, the faster kind
+                // for two steps per launch).
+                if (!(f & DP_TILE_NONTEMPORAL))
+                    for (int w : stack_wave_choices)
+                        for (int r : stack_row_choices) {
+                            const int sf = f | dp_sim::kTwoStepTileFlags | DP_TILE_STACK | (w << 16);
+                            sweep.push_back({r, d, sf, time_shape(r, d, sf, trials)});
+                        }
             }
         }
         // The sweep's few fastest shapes, timed again with three times the

@@ -118,6 +118,14 @@ void launch_fused_pair(const StepArgs &args, int rows, bool nontemporal, bool bu
 void launch_fused_pair_batch(const InstanceDesc *instances, const StepSlot *slots, const uint32_t *owners,
                              uint32_t n_tiles, const InjDesc *injections, int rows, bool nontemporal, bool buffer,
                              uint32_t xcd_group, hipStream_t stream);
+// Two steps per launch with `waves` (2 or 4) waves of a workgroup stacked
+// along y on one tile of waves·rows register rows (rows 4, 6 or 8), of which
+// waves·rows − 4 are owned; neighbouring waves trade edge rows through LDS.
+// One workgroup per tile.
+void launch_fused_stack(const StepArgs &args, int rows, int waves, bool buffer, hipStream_t stream);
+void launch_fused_stack_batch(const InstanceDesc *instances, const StepSlot *slots, const uint32_t *owners,
+                              uint32_t n_tiles, const InjDesc *injections, int rows, int waves, bool buffer,
+                              uint32_t xcd_group, hipStream_t stream);
 // One launch advancing every active instance of a batch by one step.
 // `blocks` is the persistent grid size (workgroups of 4 waves); with
 // `direct` the launch instead has one wave per tile.

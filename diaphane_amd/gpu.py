@@ -24,6 +24,7 @@ TILE_XCD_REMAP = 1
 TILE_NONTEMPORAL = 2
 TILE_BUFFER = 4
 TILE_TWO_STEP = 8
+TILE_STACK = 32  # with TILE_TWO_STEP: waves stacked along y; waves (2 or 4) in bits 16..19
 
 BUFFER_ELECTRIC = 0
 BUFFER_MAGNETIC = 1

@@ -330,7 +330,8 @@ enum { DP_KERNEL_FUSED = 0, DP_KERNEL_TWO_PASS = 1, DP_KERNEL_FUSED_OVERLAP = 2 
 /* DP_KERNEL_FUSED uses 64-aligned tiles whose x halo is fetched by the edge
  * lanes; DP_KERNEL_FUSED_OVERLAP uses 62-wide tiles whose halo columns are
  * ordinary lanes. */
-enum { DP_TILE_XCD_REMAP = 1, DP_TILE_NONTEMPORAL = 2, DP_TILE_BUFFER = 4, DP_TILE_TWO_STEP = 8 };
+enum { DP_TILE_XCD_REMAP = 1, DP_TILE_NONTEMPORAL = 2, DP_TILE_BUFFER = 4, DP_TILE_TWO_STEP = 8,
+       DP_TILE_STACK = 32 };
 int dp_sim_set_kernel(dp_sim *sim, int kernel);
 /* Fused-kernel tiling: rows per wave along y (4, 6, 8, 10 or 12), z-slab
  * depth in planes, and flags:
@@ -348,18 +349,27 @@ int dp_sim_set_kernel(dp_sim *sim, int kernel);
  *                         columns and rows − 4 of their rows (6, 8 or 10;
  *                         other values select 8); an odd step left over in
  *                         an advance runs single.
+ *   DP_TILE_STACK         with DP_TILE_TWO_STEP: the waves of a workgroup
+ *                         (W in bits 16..19: 2 or 4, 0 meaning 4) stack
+ *                         along y on one tile of W·rows register rows
+ *                         (rows 4 or 6; other values select 6), of which
+ *                         W·rows − 4 are owned; neighbouring waves trade
+ *                         their edge rows through LDS at three barriers per
+ *                         plane instead of recomputing them.
  * A new sim of at least 32 Mi cells whose offsets fit buffer addressing
- * starts with two steps per launch (10 rows, buffer addressing, XCD runs of
- * 8); smaller ones with one step (4 rows, non-temporal stores).
+ * starts with two steps per launch on stacked tiles (four waves of 4 rows,
+ * buffer addressing, XCD runs of 8); smaller ones with one step (4 rows,
+ * non-temporal stores).
  * 0 for rows selects 4; 0 for depth selects a size-aware default (the power
  * of two at or below the number of x-y tiles / 50, within 1..32, or / 25
  * within 2..32 for two steps per launch: short slabs give a small grid
- * enough waves, long ones amortize a large grid's halo planes). flags < 0 keeps the current flags. Results never depend on any of
- * it. */
+ * enough waves, long ones amortize a large grid's halo planes). flags < 0
+ * keeps the current flags. Results never depend on any of it. */
 int dp_sim_set_tile(dp_sim *sim, int rows, int depth, int flags);
 /* Autotuner for DP_KERNEL_FUSED: times `trials` launches of every rows
  * (4, 6, 8 one step per launch with non-temporal stores; 6, 8, 10 two steps
- * per launch with either store kind and XCD runs of 8) × depth (1 to 64 in powers of two, up
+ * per launch with either store kind and XCD runs of 8; 4 or 6 rows in 2 or 4
+ * stacked waves with cached stores and XCD runs of 8) × depth (1 to 64 in powers of two, up
  * to twice the grid's z extent) × addressing (pointer, buffer where it fits)
  * candidate on this sim's grid, times the four fastest again with 3·trials
  * launches in two alternating rounds, keeps the fastest per step and reports
@@ -465,17 +475,23 @@ int dp_batch_set_graph_steps(dp_batch *batch, uint32_t steps);
  * count runs alone in its launch). With DP_TILE_TWO_STEP, DP_TILE_XCD_REMAP
  * (run length G in bits 8..15, 0 meaning 8) gives each XCD runs of G
  * consecutive workgroups of the tile list; the one-step batch kernels ignore
- * it. The default is two steps per launch, 6 rows, DP_TILE_BUFFER, cached
- * stores, XCD runs of 4. The autotuner keeps the XCD bits as they are. */
+ * it. DP_TILE_STACK (with DP_TILE_TWO_STEP; waves in bits 16..19, 2 or 4)
+ * stacks the waves of a workgroup along y on one tile, as for a single sim
+ * (rows 4 or 6). The default is two steps per launch, 6 rows, DP_TILE_BUFFER,
+ * cached stores, XCD runs of 4. The autotuner keeps the XCD bits as they
+ * are. */
 enum { DP_BATCH_WAVE_PER_TILE = 16 };
 int dp_batch_set_tile(dp_batch *batch, int rows, int depth, int flags);
 /* Autotuner: times `trials` launches of every grid form × addressing ×
  * rows × depth candidate (persistent, one wave per tile, or two steps per
- * launch with either store kind; pointer or buffer; rows 4, 6, 8, or 6, 8,
- * 10 for two steps; depth 8, 16, 32, 64) on the members as they are, keeps the fastest per step and
- * reports rows, depth and time (any out pointer may be null; dp_batch_tiling
- * reports the form). Member state is unchanged: trial launches write only
- * the idle half of each ping-pong pair. Synchronous. */
+ * launch with either store kind, or two steps on 2 or 4 stacked waves with
+ * buffer addressing; pointer or buffer; rows 4, 6, 8, or 6, 8, 10 for two
+ * steps, 4 or 6 stacked; depth 8, 16, 32, 64) on the members as they are,
+ * times the four fastest again with 3·trials launches in two alternating
+ * rounds, keeps the fastest per step and reports rows, depth and time (any
+ * out pointer may be null; dp_batch_tiling reports the form). Member state
+ * is unchanged: trial launches write only the idle half of each ping-pong
+ * pair. Synchronous. */
 int dp_batch_autotune(dp_batch *batch, uint32_t trials, int *rows, int *depth, float *ms_per_step);
 /* The tiling in use (any out pointer may be null). */
 int dp_batch_tiling(const dp_batch *batch, int *rows, int *depth, int *flags);
@@ -523,10 +539,15 @@ typedef struct dp_request_info {
     /* seconds since the server was created; 0 until reached */
     double submitted_s, admitted_s, started_s, completed_s;
 } dp_request_info;
+/* busy_s: loop time spent advancing and waiting for chunks; kernel_s: the
+ * device time of those chunks' launches (first launch to last, by events);
+ * cell_updates: cells × steps advanced, over all requests. */
 typedef struct dp_server_stats {
     uint32_t queued, preparing, ready, running;
     uint64_t used_bytes, budget_bytes, chunks;
     double busy_s, uptime_s;
+    double kernel_s;
+    uint64_t cell_updates;
 } dp_server_stats;
 int dp_server_create(dp_engine *engine, const dp_server_config *config, dp_server **out);
 /* Stops the loop; every request still held is dropped. */

@@ -79,7 +79,7 @@ def test_python_constants_match_the_header():
         "DP_KERNEL_FUSED": gpu.KERNEL_FUSED, "DP_KERNEL_TWO_PASS": gpu.KERNEL_TWO_PASS,
         "DP_KERNEL_FUSED_OVERLAP": gpu.KERNEL_FUSED_OVERLAP,
         "DP_TILE_XCD_REMAP": gpu.TILE_XCD_REMAP, "DP_TILE_NONTEMPORAL": gpu.TILE_NONTEMPORAL,
-        "DP_TILE_BUFFER": gpu.TILE_BUFFER, "DP_TILE_TWO_STEP": gpu.TILE_TWO_STEP,
+        "DP_TILE_BUFFER": gpu.TILE_BUFFER, "DP_TILE_TWO_STEP": gpu.TILE_TWO_STEP, "DP_TILE_STACK": gpu.TILE_STACK,
         "DP_BUFFER_ELECTRIC": gpu.BUFFER_ELECTRIC, "DP_BUFFER_MAGNETIC": gpu.BUFFER_MAGNETIC,
         "DP_BUFFER_INTENSITY": gpu.BUFFER_INTENSITY, "DP_BUFFER_MATERIAL_INDEX": gpu.BUFFER_MATERIAL_INDEX,
         "DP_BUFFER_INTENSITY_F32": gpu.BUFFER_INTENSITY_F32, "DP_BUFFER_LOOKUP": gpu.BUFFER_LOOKUP,

@@ -15,7 +15,7 @@ import oracle
 from diaphane_amd import Extent, Scene
 from diaphane_amd import _lib
 from diaphane_amd.batch import Batch
-from diaphane_amd.gpu import TILE_BUFFER, TILE_NONTEMPORAL, TILE_TWO_STEP, TILE_XCD_REMAP, Simulation
+from diaphane_amd.gpu import TILE_BUFFER, TILE_NONTEMPORAL, TILE_STACK, TILE_TWO_STEP, TILE_XCD_REMAP, Simulation
 from diaphane_amd.workloads import batched_uneven
 
 from test_gpu_batch import assert_matches_oracle, bits_equal, mixed_scenes, oracle_seeded, scene_name, seeded
@@ -24,7 +24,10 @@ pytestmark = pytest.mark.gpu
 
 SHAPES = [(8, 0, TILE_TWO_STEP | TILE_NONTEMPORAL), (6, 16, TILE_TWO_STEP), (10, 9, TILE_TWO_STEP | TILE_BUFFER),
           # XCD-grouped workgroup order: runs of 8, and of 3 (a tail past the last whole group)
-          (6, 0, TILE_TWO_STEP | TILE_BUFFER | TILE_XCD_REMAP), (8, 5, TILE_TWO_STEP | TILE_XCD_REMAP | (3 << 8))]
+          (6, 0, TILE_TWO_STEP | TILE_BUFFER | TILE_XCD_REMAP), (8, 5, TILE_TWO_STEP | TILE_XCD_REMAP | (3 << 8)),
+          # waves stacked along y
+          (4, 0, TILE_TWO_STEP | TILE_BUFFER | TILE_STACK | TILE_XCD_REMAP | (4 << 8)),
+          (6, 7, TILE_TWO_STEP | TILE_STACK | (2 << 16))]
 
 
 @pytest.mark.parametrize("graph_steps", [16, 0])

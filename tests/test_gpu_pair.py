@@ -14,14 +14,19 @@ import pytest
 
 import oracle
 from diaphane_amd import Extent, Scene
-from diaphane_amd.gpu import TILE_BUFFER, TILE_NONTEMPORAL, TILE_TWO_STEP, TILE_XCD_REMAP, Simulation
+from diaphane_amd.gpu import TILE_BUFFER, TILE_NONTEMPORAL, TILE_STACK, TILE_TWO_STEP, TILE_XCD_REMAP, Simulation
 
 pytestmark = pytest.mark.gpu
 
 SHAPES = [(6, 1, TILE_NONTEMPORAL | TILE_TWO_STEP), (8, 3, TILE_TWO_STEP), (10, 16, TILE_BUFFER | TILE_TWO_STEP),
           (8, 0, TILE_BUFFER | TILE_NONTEMPORAL | TILE_TWO_STEP),
           # XCD-grouped workgroups: runs of 8, and of 3 (a tail past the last whole group)
-          (10, 4, TILE_BUFFER | TILE_TWO_STEP | TILE_XCD_REMAP), (6, 2, TILE_TWO_STEP | TILE_XCD_REMAP | (3 << 8))]
+          (10, 4, TILE_BUFFER | TILE_TWO_STEP | TILE_XCD_REMAP), (6, 2, TILE_TWO_STEP | TILE_XCD_REMAP | (3 << 8)),
+          # waves stacked along y: 4 waves of 6 rows (20 owned), 2 of 4 (4 owned), pointer addressing
+          (6, 0, TILE_BUFFER | TILE_TWO_STEP | TILE_STACK | TILE_XCD_REMAP),
+          (4, 3, TILE_BUFFER | TILE_TWO_STEP | TILE_STACK | (2 << 16)),
+          (4, 5, TILE_TWO_STEP | TILE_STACK | (4 << 16) | TILE_XCD_REMAP | (3 << 8)),
+          (6, 1, TILE_TWO_STEP | TILE_STACK | (2 << 16))]
 SHAPE_IDS = [f"rows{r}-depth{d}-flags{f}" for r, d, f in SHAPES]
 
 

@@ -14,7 +14,7 @@ import pytest
 import oracle
 from diaphane_amd.batch import Batch
 from diaphane_amd.gpu import KERNEL_FUSED, KERNEL_FUSED_OVERLAP, KERNEL_TWO_PASS, TILE_BUFFER, TILE_NONTEMPORAL, \
-    TILE_TWO_STEP, TILE_XCD_REMAP, Simulation
+    TILE_STACK, TILE_TWO_STEP, TILE_XCD_REMAP, Simulation
 
 from random_scenes import random_case
 
@@ -39,6 +39,8 @@ def run_plan(seed: int):
         flags |= TILE_XCD_REMAP | (int(rng.choice([0, 2, 4])) << 8)
     if rng.random() < 0.5:
         flags |= TILE_TWO_STEP  # two steps per launch (the fused kernel's option)
+    if flags & TILE_TWO_STEP and rng.random() < 0.5:
+        flags |= TILE_STACK | (int(rng.choice([0, 2, 4])) << 16)  # waves stacked along y
     return kernel, rows, depth, flags, bool(seed % 2)
 
 
