@@ -107,6 +107,7 @@ def _declare(lib: ctypes.CDLL) -> None:
         "dp_sim_stream_probe": (c_int, [sim, c_int, POINTER(c_double)]),
         "dp_sim_tile_probe": (c_int, [sim, c_int, c_int, c_int, POINTER(c_double)]),
         "dp_sim_measure_peak": (c_int, [sim, POINTER(c_float)]),
+        "dp_sim_measure_peak_noalloc": (c_int, [sim, POINTER(c_float)]),
         "dp_timeline_create": (c_int, [c_uint64, c_uint32, POINTER(c_void_p)]),
         "dp_timeline_destroy": (None, [c_void_p]),
         "dp_timeline_observe": (c_int, [c_void_p, sim]),

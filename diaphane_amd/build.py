@@ -37,6 +37,7 @@ SOURCES = [
     ("kernels.hip", DEVICE),
     ("kernels_pair.hip", DEVICE),
     ("kernels_stack.hip", DEVICE),
+    ("noalloc.hip", DEVICE),
     ("scene.cpp", HOST),
     ("scene_ron.cpp", HOST),
     ("timeline.cpp", HOST),

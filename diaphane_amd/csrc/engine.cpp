@@ -184,6 +184,7 @@ int dp_engine_create(int device, dp_engine **out) {
         check(hipGetDeviceCount(&count), "hipGetDeviceCount");
         if (device < 0 || device >= count) fail(DP_ERR_DEVICE, "no usable GPU device");
         check(hipSetDevice(device), "hipSetDevice");
+        prepare_noalloc_scratch(device);
         hipDeviceProp_t prop;
         check(hipGetDeviceProperties(&prop, device), "hipGetDeviceProperties");
         auto engine = std::make_unique<dp_engine>();

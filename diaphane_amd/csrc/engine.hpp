@@ -111,6 +111,13 @@ struct PinnedBuffer {
 
 using namespace dpa_detail;
 
+namespace dpa {
+// Takes the device address of the fixed scratch array that
+// dp_sim_measure_peak_noalloc uses, once per device (noalloc.hip); called
+// by dp_engine_create with `device` current.
+void prepare_noalloc_scratch(int device);
+}  // namespace dpa
+
 struct dp_batch;
 
 struct dp_engine {

@@ -299,6 +299,17 @@ int dp_sim_write_fields(dp_sim *sim, const float *electric,
  * sum (rounded to f32) over the accumulated step count, 0 with no sum.
  * Synchronous. */
 int dp_sim_measure_peak(dp_sim *sim, float peak[8]);
+/* The same eight maxima, bit for bit, from a call that allocates nothing.
+ * No counterpart in the reference: for callers that cannot allocate (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * allocation). The per-block partials and the final maxima live in a fixed
+ * file-scope device array of 128 KiB + 32 B per GPU, held until the process
+ * ends; its device address is taken once per device by dp_engine_create.
+ * The reduction finishes on the device, so no host buffer is allocated
+ * either. DP_ERR_SIZE if a grid would need more than its 4096 partial
+ * slots (the peak kernel never uses more). Calls that overlap on one GPU
+ * share the array and race: serialize them. Synchronous. */
+int dp_sim_measure_peak_noalloc(dp_sim *sim, float peak[8]);
 
 /* cpu::Simulation::energy (src/cpu.rs:654-687) computed on the device:
  * per-component volumes from the primary/dual widths, f64 accumulation. */
