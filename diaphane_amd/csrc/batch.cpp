@@ -168,6 +168,7 @@ struct dp_batch {
         for (const dp_sim *s : members)
             tiles_xy += uint64_t((s->grid.extent[0] + owned_x() - 1) / owned_x()) *
                         ((s->grid.extent[1] + owned_rows() - 1) / owned_rows());
+        if (stack()) tiles_xy *= uint64_t(waves());  // the rule counts waves
         const uint32_t capped = static_cast<uint32_t>(std::min<uint64_t>(tiles_xy, 0xFFFFFFFFull));
         return two_step() ? std::min<uint32_t>(32, dp_sim::default_pair_depth(capped))
                           : std::min<uint32_t>(16, dp_sim::default_depth(capped));

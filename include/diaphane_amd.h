@@ -367,7 +367,7 @@ int dp_sim_set_kernel(dp_sim *sim, int kernel);
  *                         W·rows − 4 are owned; neighbouring waves trade
  *                         their edge rows through LDS at three barriers per
  *                         plane instead of recomputing them.
- * A new sim of at least 32 Mi cells whose offsets fit buffer addressing
+ * A new sim of at least 768 Ki cells whose offsets fit buffer addressing
  * starts with two steps per launch on stacked tiles (four waves of 4 rows,
  * buffer addressing, XCD runs of 8); smaller ones with one step (4 rows,
  * non-temporal stores).
