@@ -43,12 +43,17 @@ def main(which) -> int:
     cases = {
         "graded": lambda: (workloads.graded_scattering_256().scene, 2000, "graded_scattering_256 (full grid)"),
         "layered": lambda: (workloads.layered_768(nz=64).scene, 1000, "layered stack, 768x768x64 slab"),
+        # the same slab while the dipole's pulse is still inside it (by 1000 steps the 64-plane
+        # slab's absorbing walls have taken all but a remnant, whose peak is small)
+        "layered300": lambda: (workloads.layered_768(nz=64).scene, 300, "layered stack, 768x768x64 slab"),
     }
     for key in which or list(cases):
         scene, steps, label = cases[key]()
         r = error_of(scene, steps, threads)
+        both = max(r["E_peak"], r["H_peak"])
         print(f"{label}: {r['cells']} cells, {steps} steps: E {r['E']:.2e} of peak {r['E_peak']:.3e}, "
-              f"H {r['H']:.2e} of peak {r['H_peak']:.3e} ({r['seconds']:.0f} s on {threads} threads)", flush=True)
+              f"H {r['H']:.2e} of peak {r['H_peak']:.3e}; of the larger peak: E {r['E'] * r['E_peak'] / both:.2e}, "
+              f"H {r['H'] * r['H_peak'] / both:.2e} ({r['seconds']:.0f} s on {threads} threads)", flush=True)
     return 0
 
 

@@ -14,7 +14,7 @@ import pytest
 import oracle
 from diaphane_amd import AXES, Axis, Boundary, Extent, Grid, Refinement, Scene, Source, Waveform
 from diaphane_amd import workloads
-from diaphane_amd.gpu import KERNEL_TWO_PASS, Simulation
+from diaphane_amd.gpu import KERNEL_TWO_PASS, TILE_BUFFER, TILE_STACK, TILE_TWO_STEP, TILE_XCD_REMAP, Simulation
 
 from conftest import assert_field_parity
 
@@ -161,10 +161,13 @@ def test_a_graded_closed_box_conserves_energy_for_20000_steps(context):
     assert bits_equal(reference.magnetic_all, device.read_magnetic())
 
 
-def test_the_largest_grid_takes_the_64_bit_path(context):
+@pytest.mark.parametrize("tile", [None, (4, 0, TILE_TWO_STEP | TILE_STACK | TILE_BUFFER | TILE_XCD_REMAP)],
+                         ids=["default", "stacked"])
+def test_the_largest_grid_takes_the_64_bit_path(context, tile):
     # 1024 × 1024 × 1025 cells: one field component is 4.3 GB (> 4 GiB), so
-    # the fused kernel addresses with 64-bit offsets. A dipole off-centre, a
-    # few steps, every cell against the oracle.
+    # the fused kernels address with 64-bit offsets (buffer addressing, asked
+    # for, does not fit and is not used). A dipole off-centre, three steps
+    # (with stacked tiles: a pair and a single), every cell against the oracle.
     grid = Grid(Extent(1024, 1024, 1025), 1e-3)
     scene = Scene(grid, Boundary.PEC)
     frequency = grid.frequency_for_resolution(16.0)
@@ -172,6 +175,8 @@ def test_the_largest_grid_takes_the_64_bit_path(context):
     cells = 1024 * 1024 * 1025
     assert cells * 4 > 0xFFFFFFFF
     device = Simulation(context, scene)
+    if tile:
+        device.set_tile(*tile)
     device.advance_by(3)
     reference = oracle.Simulation(scene)
     reference.advance_by(3)
