@@ -119,7 +119,7 @@ void launch_fused_pair_batch(const InstanceDesc *instances, const StepSlot *slot
                              uint32_t n_tiles, const InjDesc *injections, int rows, bool nontemporal, bool buffer,
                              uint32_t xcd_group, hipStream_t stream);
 // Two steps per launch with `waves` (2 or 4) waves of a workgroup stacked
-// along y on one tile of waves·rows register rows (rows 4, 6 or 8), of which
+// along y on one tile of waves·rows register rows (rows 4 or 6), of which
 // waves·rows − 4 are owned; neighbouring waves trade edge rows through LDS.
 // One workgroup per tile.
 void launch_fused_stack(const StepArgs &args, int rows, int waves, bool buffer, hipStream_t stream);

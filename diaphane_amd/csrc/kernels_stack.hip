@@ -19,7 +19,6 @@ static void launch_stack_as(const StepArgs &args, int rows, hipStream_t stream) 
     const uint32_t blocks = args.tiles_total;
     switch (rows) {
     case 4: k_fused_stack<4, W, Access<false>, kIntensity><<<blocks, W * kLanes, 0, stream>>>(args); break;
-    case 8: k_fused_stack<8, W, Access<false>, kIntensity><<<blocks, W * kLanes, 0, stream>>>(args); break;
     default: k_fused_stack<6, W, Access<false>, kIntensity><<<blocks, W * kLanes, 0, stream>>>(args); break;
     }
 }
@@ -55,10 +54,6 @@ static void launch_stack_batch_as(const InstanceDesc *instances, const StepSlot 
     switch (rows) {
     case 4:
         k_fused_stack_batch<4, W, Access<false>><<<n_tiles, W * kLanes, 0, stream>>>(instances, slots, owners,
-                                                                                    n_tiles, injections, xcd_group);
-        break;
-    case 8:
-        k_fused_stack_batch<8, W, Access<false>><<<n_tiles, W * kLanes, 0, stream>>>(instances, slots, owners,
                                                                                     n_tiles, injections, xcd_group);
         break;
     default:
