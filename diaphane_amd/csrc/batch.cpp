@@ -150,11 +150,19 @@ struct dp_batch {
 
     uint32_t owned_x() const { return two_step() ? kPairOwnedX : kAlignedX; }
     int owned_rows() const { return stack() ? waves() * rows - 4 : rows - (two_step() ? 4 : 2); }
+    // Column and row tiles of a member's grid (two-step tiles own more
+    // columns at the walls, stacked ones more rows).
+    uint32_t tiles_x_of(uint32_t nx) const {
+        return two_step() ? pair_tiles_x(nx) : (nx + owned_x() - 1) / owned_x();
+    }
+    uint32_t tiles_y_of(uint32_t ny) const {
+        return stack() ? stack_tiles_y(ny, uint32_t(waves() * rows)) : (ny + owned_rows() - 1) / owned_rows();
+    }
 
     void tiles_of(const dp_sim *sim, InstanceDesc &d) const {
         const uint32_t nx = sim->grid.extent[0], ny = sim->grid.extent[1], nz = sim->grid.extent[2];
-        d.tiles_x = (nx + owned_x() - 1) / owned_x();
-        d.tiles_y = (ny + owned_rows() - 1) / owned_rows();
+        d.tiles_x = tiles_x_of(nx);
+        d.tiles_y = tiles_y_of(ny);
         d.depth = std::min<uint32_t>(nz, slab);
         d.tiles_z = (nz + d.depth - 1) / d.depth;
         d.tiles_total = d.tiles_x * d.tiles_y * d.tiles_z;
@@ -166,8 +174,7 @@ struct dp_batch {
     uint32_t auto_depth() const {
         uint64_t tiles_xy = 0;
         for (const dp_sim *s : members)
-            tiles_xy += uint64_t((s->grid.extent[0] + owned_x() - 1) / owned_x()) *
-                        ((s->grid.extent[1] + owned_rows() - 1) / owned_rows());
+            tiles_xy += uint64_t(tiles_x_of(s->grid.extent[0])) * tiles_y_of(s->grid.extent[1]);
         if (stack()) tiles_xy *= uint64_t(waves());  // the rule counts waves
         const uint32_t capped = static_cast<uint32_t>(std::min<uint64_t>(tiles_xy, 0xFFFFFFFFull));
         return two_step() ? std::min<uint32_t>(32, dp_sim::default_pair_depth(capped))

@@ -352,8 +352,9 @@ struct dp_sim {
                     uint32_t waves_per_tile = 1) const {
         const uint32_t nx = grid.extent[0], ny = grid.extent[1], nz = grid.extent[2];
         const uint32_t owned_y = uint32_t(r - halo_rows);
-        a.tiles_x = (nx + owned_x - 1) / owned_x;
-        a.tiles_y = (ny + owned_y - 1) / owned_y;
+        // two-step tiles own more columns at the walls, stacked ones more rows
+        a.tiles_x = owned_x == uint32_t(kPairOwnedX) ? pair_tiles_x(nx) : (nx + owned_x - 1) / owned_x;
+        a.tiles_y = waves_per_tile > 1 ? stack_tiles_y(ny, uint32_t(r)) : (ny + owned_y - 1) / owned_y;
         const uint32_t tiles_xy = a.tiles_x * a.tiles_y;
         const uint32_t waves_xy = tiles_xy * waves_per_tile;
         a.depth = std::min<uint32_t>(
