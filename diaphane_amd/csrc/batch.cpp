@@ -150,11 +150,9 @@ struct dp_batch {
 
     uint32_t owned_x() const { return two_step() ? kPairOwnedX : kAlignedX; }
     int owned_rows() const { return stack() ? waves() * rows - 4 : rows - (two_step() ? 4 : 2); }
-    // Column and row tiles of a member's grid (two-step tiles own more
-    // columns at the walls, stacked ones more rows).
-    uint32_t tiles_x_of(uint32_t nx) const {
-        return two_step() ? pair_tiles_x(nx) : (nx + owned_x() - 1) / owned_x();
-    }
+    // Column and row tiles of a member's grid (stacked two-step tiles own
+    // more rows at the walls).
+    uint32_t tiles_x_of(uint32_t nx) const { return (nx + owned_x() - 1) / owned_x(); }
     uint32_t tiles_y_of(uint32_t ny) const {
         return stack() ? stack_tiles_y(ny, uint32_t(waves() * rows)) : (ny + owned_rows() - 1) / owned_rows();
     }

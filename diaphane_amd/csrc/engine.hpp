@@ -352,8 +352,8 @@ struct dp_sim {
                     uint32_t waves_per_tile = 1) const {
         const uint32_t nx = grid.extent[0], ny = grid.extent[1], nz = grid.extent[2];
         const uint32_t owned_y = uint32_t(r - halo_rows);
-        // two-step tiles own more columns at the walls, stacked ones more rows
-        a.tiles_x = owned_x == uint32_t(kPairOwnedX) ? pair_tiles_x(nx) : (nx + owned_x - 1) / owned_x;
+        // stacked two-step tiles own more rows at the walls
+        a.tiles_x = (nx + owned_x - 1) / owned_x;
         a.tiles_y = waves_per_tile > 1 ? stack_tiles_y(ny, uint32_t(r)) : (ny + owned_y - 1) / owned_y;
         const uint32_t tiles_xy = a.tiles_x * a.tiles_y;
         const uint32_t waves_xy = tiles_xy * waves_per_tile;

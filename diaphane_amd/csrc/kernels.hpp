@@ -111,15 +111,6 @@ bool buffer_addressing_fits(uint64_t cells, uint32_t nx, uint32_t ny);
 // Two steps per launch: 64-lane waves owning 60 columns (two halo lanes on
 // each side) and rows − 4 of `rows` register rows (6, 8 or 10).
 constexpr int kPairOwnedX = kLanes - 4;
-// Column tiles of the two-step kernels. Tile t's lane 0 sits at x = 60·t.
-// Nothing lies below the low wall, so tile 0 also owns its lanes 0 and 1;
-// the last tile owns every lane from 2 that lies inside the grid, which the
-// count keeps valid by leaving at least one of its lanes past the high wall
-// (that lane's zero is the field beyond the wall). Interior tiles own lanes
-// 2..61.
-inline uint32_t pair_tiles_x(uint32_t nx) {
-    return nx < uint32_t(kLanes) ? 1u : (nx - 3 + kPairOwnedX - 1) / kPairOwnedX;
-}
 // Row tiles of stacked two-step tiles of `rows` rows in all (waves × rows per
 // wave). Tile t's first row sits at y = (rows − 4)·t. Every half-step of a
 // stacked tile runs on all its rows, so tile 0 also owns its first two rows
