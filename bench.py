@@ -426,6 +426,10 @@ def served_summary(run: dict, rate: float, chunk_steps: int) -> dict:
         "sequential_kernel_rate": run["sequential"]["kernel_rate"],
         "chunk_steps": chunk_steps,
         "chunk_cells": run["chunk_cells"],
+        # No schedule can finish before the last request arrives: the stream's
+        # cell-updates over its arrival span bound any served rate.
+        "arrival_bound_value": run["cell_updates"] / max(run["stream"][-1].arrival_s - run["stream"][0].arrival_s,
+                                                         1e-9),
     }
 
 
@@ -683,6 +687,9 @@ def main() -> int:
             result["batched"] = batched_summary(run_batched(context, 1000, args.warmup))
         if not args.no_served:
             result["served"] = served_summary(run_served(context, 200, 50.0, 32, 0, 0), 50.0, 32)
+            # the top of the stated arrival range, where the device, not the
+            # arrivals, bounds the served rate
+            result["served_200"] = served_summary(run_served(context, 200, 200.0, 32, 0, 0), 200.0, 32)
     if not args.no_cpu:
         cpu = cpu_baseline(args.workload, args.cpu_seconds)
         result["cpu_baseline"] = cpu

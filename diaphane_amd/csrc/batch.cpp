@@ -61,14 +61,16 @@ struct dp_batch {
     uint64_t version = 0;
 
     // Defaults measured on MI355X over 256 uneven grids (48–160 cells a
-    // side): two steps per launch with 6 rows, buffer addressing and cached
-    // stores is the autotuner's pick, 1.6x the best one-step form (one wave
-    // per tile, 4 rows, 16-plane slabs). Workgroups go to the XCDs in runs of
-    // 4 (16 consecutive tiles, y-neighbours sharing one L2): 1.5% faster than
-    // the hardware's round robin there (runs of 2, 8 and 16: 0.9–1.2%).
+    // side): two steps per launch with 8 rows, buffer addressing and cached
+    // stores is the autotuner's pick (1.84–1.89 ms a step against 1.95–1.98
+    // with 6 rows; on the served stream's 200 grids, 32–256 a side, 2.35
+    // against 2.40 ms), 1.6x the best one-step form (one wave per tile, 4
+    // rows, 16-plane slabs). Workgroups go to the XCDs in runs of 4 (16
+    // consecutive tiles, y-neighbours sharing one L2): 1.5% faster than the
+    // hardware's round robin there (runs of 2, 8 and 16: 0.9–1.2%).
     // depth 0 sizes the slabs to the membership (auto_depth): a batch of one
     // small grid needs short slabs to put enough waves on the device.
-    int rows = 6;
+    int rows = 8;
     uint32_t depth = 0;
     uint32_t slab = 16;  // the slab depth the current tables use
     int flags = DP_BATCH_WAVE_PER_TILE | DP_TILE_TWO_STEP | DP_TILE_BUFFER | DP_TILE_XCD_REMAP | (4 << 8);
@@ -599,7 +601,7 @@ int dp_batch_set_tile(dp_batch *batch, int rows, int depth, int flags) {
                                          : "rows must be 4, 6 or 8");
         batch->sync();
         batch->flags = new_flags;
-        batch->rows = rows ? rows : (stacked ? 4 : two ? 6 : 4);
+        batch->rows = rows ? rows : (stacked ? 4 : two ? 8 : 4);
         batch->depth = static_cast<uint32_t>(depth);
         batch->dirty = true;
     });

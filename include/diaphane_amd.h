@@ -475,7 +475,7 @@ int dp_batch_wait(dp_batch *batch, uint32_t timeout_ms);
  * directly). Results never depend on it. */
 int dp_batch_set_graph_steps(dp_batch *batch, uint32_t steps);
 /* Rows per wave (4, 6 or 8; with DP_TILE_TWO_STEP 6, 8 or 10; 0 = default
- * 4, or 6), z-slab depth (0, the default: sized to the membership, the
+ * 4, or 8 with DP_TILE_TWO_STEP, 4 with DP_TILE_STACK), z-slab depth (0, the default: sized to the membership, the
  * single-sim rule over all members' x-y tiles together, at most 16, or 32
  * for two steps) and
  * flags (< 0 keeps the current ones): DP_TILE_NONTEMPORAL, DP_TILE_BUFFER
@@ -488,7 +488,7 @@ int dp_batch_set_graph_steps(dp_batch *batch, uint32_t steps);
  * consecutive workgroups of the tile list; the one-step batch kernels ignore
  * it. DP_TILE_STACK (with DP_TILE_TWO_STEP; waves in bits 16..19, 2 or 4)
  * stacks the waves of a workgroup along y on one tile, as for a single sim
- * (rows 4 or 6). The default is two steps per launch, 6 rows, DP_TILE_BUFFER,
+ * (rows 4 or 6). The default is two steps per launch, 8 rows, DP_TILE_BUFFER,
  * cached stores, XCD runs of 4. The autotuner keeps the XCD bits as they
  * are. */
 enum { DP_BATCH_WAVE_PER_TILE = 16 };
