@@ -127,18 +127,19 @@ def batch_cpu_baseline(instances, seconds: float) -> dict:
     }
 
 
-def run_batched(context, steps: int, warmup: int, tile=None) -> dict:
+def run_batched(context, steps: int, warmup: int, tile=None, members: str = "all", sequential: bool = True) -> dict:
     """The 256 uneven solves run one at a time, then as one batch from the
     same initial fields; eight members are compared bit for bit between the
     two runs. Returns the measurements (tile: (rows, depth, flags) or None
-    for the autotuner)."""
+    for the autotuner; members: "all", or only the "pec" cavities or the
+    "absorbing" boxes; sequential False skips the one-at-a-time run)."""
     import numpy as np
 
     from diaphane_amd import workloads
     from diaphane_amd.batch import Batch
     from diaphane_amd.gpu import Simulation
 
-    instances = workloads.batched_uneven_256()
+    instances = [i for i in workloads.batched_uneven_256() if members in ("all", i.scene.boundary.kind)]
     log(f"creating {len(instances)} sims")
     sims = []
     initial = []
@@ -154,21 +155,21 @@ def run_batched(context, steps: int, warmup: int, tile=None) -> dict:
     total_cells = sum(cells)
 
     # One at a time: each solve runs its steps alone, then the next.
-    log("one at a time")
-    if warmup:
+    log("one at a time" if sequential else "one at a time: skipped")
+    if warmup and sequential:
         sims[0].advance_by(warmup)
         sims[0].wait()
         sims[0].write_fields(*initial[0], 0)
     seq_kernel_ms = 0.0
     t0 = time.perf_counter()
-    for sim in sims:
+    for sim in sims if sequential else []:
         sim.record_event(0)
         sim.advance_by(steps)
         sim.record_event(1)
         sim.wait()
         seq_kernel_ms += sim.elapsed_ms(0, 1)
     seq_wall = time.perf_counter() - t0
-    check = list(range(0, len(sims), max(1, len(sims) // 8)))
+    check = list(range(0, len(sims), max(1, len(sims) // 8))) if sequential else []
     solo_fields = {k: (sims[k].read_electric(), sims[k].read_magnetic()) for k in check}
 
     log("batched")
@@ -204,7 +205,8 @@ def run_batched(context, steps: int, warmup: int, tile=None) -> dict:
 
     updates = total_cells * steps
     achieved = BYTES_PER_CELL * updates / (batch_ms / 1e3) / 1e9
-    log(f"  batched {updates / batch_wall / 1e9:.1f} G/s, one at a time {updates / seq_wall / 1e9:.1f} G/s")
+    log(f"  batched {updates / batch_wall / 1e9:.1f} G/s" +
+        (f", one at a time {updates / seq_wall / 1e9:.1f} G/s" if sequential else ""))
     return {
         "instances": instances,
         "value": updates / batch_wall,
@@ -219,9 +221,9 @@ def run_batched(context, steps: int, warmup: int, tile=None) -> dict:
             "wall_s": seq_wall,
             "kernel_s": seq_kernel_ms / 1e3,
             "note": "the same solves from the same initial fields, dp_sim_advance one solve after another",
-        },
-        "speedup_vs_sequential": seq_wall / batch_wall,
-        "members_bit_identical_to_solo": identical,
+        } if sequential else None,
+        "speedup_vs_sequential": seq_wall / batch_wall if sequential else None,
+        "members_bit_identical_to_solo": identical if sequential else None,
         "members_checked": len(check),
         "tiling": tuned,
         "kernel": ("k_fused_stack_batch" if tuned.get("stack_waves") else "k_fused_pair_batch")
@@ -263,7 +265,7 @@ def bench_batched(args) -> int:
 
     steps = args.steps if args.steps is not None else 1000
     tile = (args.rows, args.depth, args.tile_flags) if args.rows else None
-    run = run_batched(headless_context(), steps, args.warmup, tile)
+    run = run_batched(headless_context(), steps, args.warmup, tile, args.members, not args.no_sequential)
     value = run["value"]
     result = {
         "metric": "aggregate Yee cell-updates/s, batched vs one at a time",
@@ -284,6 +286,7 @@ def bench_batched(args) -> int:
             "cells": run["cells"],
             "cells_min_max": run["cells_min_max"],
             "pec_instances": run["pec_instances"],
+            "members": args.members,
         },
         "sequential": run["sequential"],
         "speedup_vs_sequential": run["speedup_vs_sequential"],
@@ -486,6 +489,10 @@ def main() -> int:
     parser.add_argument("--no-served", action="store_true",
                         help="default workload: skip the served_stream comparison")
     parser.add_argument("--cpu-seconds", type=float, default=10.0)
+    parser.add_argument("--members", choices=("all", "pec", "absorbing"), default="all",
+                        help="batched_uneven_256: all members, or only the PEC cavities or the absorbing boxes")
+    parser.add_argument("--no-sequential", action="store_true",
+                        help="batched_uneven_256: skip the one-at-a-time run")
     parser.add_argument("--requests", type=int, default=200, help="served_stream: number of requests")
     parser.add_argument("--rate", type=float, default=50.0, help="served_stream: arrivals per second")
     parser.add_argument("--chunk-steps", type=int, default=32, help="served_stream: launches between joins")
