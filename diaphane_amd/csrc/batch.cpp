@@ -1,5 +1,6 @@
 // The batched form: many independent sims of uneven sizes advanced together
-// by one persistent kernel launch per step over a packed instance table.
+// by one kernel launch per step, or per two steps, over a packed instance
+// table.
 //
 // A batch borrows ordinary dp_sims. While a sim is a member, the batch
 // advances it (its own advance entry point refuses) and its results are read
