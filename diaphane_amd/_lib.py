@@ -92,6 +92,8 @@ def _declare(lib: ctypes.CDLL) -> None:
         "dp_sim_accumulated_steps": (c_uint64, [sim]),
         "dp_sim_read_electric": (c_int, [sim, POINTER(c_float), c_size_t]),
         "dp_sim_read_magnetic": (c_int, [sim, POINTER(c_float), c_size_t]),
+        "dp_sim_begin_read": (c_int, [sim, c_int, POINTER(c_float), c_size_t]),
+        "dp_sim_finish_read": (c_int, [sim, c_uint32]),
         "dp_sim_read_intensity": (c_int, [sim, POINTER(c_float), c_size_t]),
         "dp_sim_read_intensity_f64": (c_int, [sim, POINTER(c_double), c_size_t]),
         "dp_sim_write_fields": (c_int, [sim, POINTER(c_float), POINTER(c_float), c_size_t, c_uint64]),

@@ -410,6 +410,41 @@ int dp_sim_read_magnetic(dp_sim *sim, float *out, size_t count) {
     return read_field(sim, sim ? sim->h : nullptr, out, count);
 }
 
+int dp_sim_begin_read(dp_sim *sim, int which, float *out, size_t count) {
+    return guarded([&] {
+        if (!sim || !out) fail(DP_ERR_INVALID, "null argument");
+        if (which != DP_BUFFER_ELECTRIC && which != DP_BUFFER_MAGNETIC)
+            fail(DP_ERR_INVALID, "which must be DP_BUFFER_ELECTRIC or DP_BUFFER_MAGNETIC");
+        if (count != 3 * sim->cells) fail(DP_ERR_SIZE, "buffer length must be 3 x cell count");
+        sim->finish_read(0);  // the staging buffer holds one readback at a time
+        if (!sim->read_stream) {
+            check(hipStreamCreateWithFlags(&sim->read_stream, hipStreamNonBlocking), "hipStreamCreate");
+            check(hipEventCreateWithFlags(&sim->read_staged, hipEventDisableTiming), "hipEventCreate");
+            check(hipEventCreateWithFlags(&sim->read_done, hipEventDisableTiming), "hipEventCreate");
+        }
+        const size_t bytes = count * sizeof(float);
+        if (sim->read_stage.bytes < bytes) sim->read_stage.allocate(bytes);
+        const DeviceBuffer &field = (which == DP_BUFFER_ELECTRIC ? sim->e : sim->h)[sim->current];
+        // On the sim's stream, after every step queued so far: the fields as
+        // of this call. The host copy then runs beside the steps that follow.
+        check(hipMemcpyAsync(sim->read_stage.ptr, field.ptr, bytes, hipMemcpyDeviceToDevice, sim->stream),
+              "hipMemcpyAsync");
+        check(hipEventRecord(sim->read_staged, sim->stream), "hipEventRecord");
+        check(hipStreamWaitEvent(sim->read_stream, sim->read_staged, 0), "hipStreamWaitEvent");
+        check(hipMemcpyAsync(out, sim->read_stage.ptr, bytes, hipMemcpyDeviceToHost, sim->read_stream),
+              "hipMemcpyAsync");
+        check(hipEventRecord(sim->read_done, sim->read_stream), "hipEventRecord");
+        sim->read_pending = true;
+    });
+}
+
+int dp_sim_finish_read(dp_sim *sim, uint32_t timeout_ms) {
+    return guarded([&] {
+        if (!sim) fail(DP_ERR_INVALID, "null sim");
+        sim->finish_read(timeout_ms);
+    });
+}
+
 int dp_sim_read_intensity_f64(dp_sim *sim, double *out, size_t count) {
     return guarded([&] {
         if (!sim || !out) fail(DP_ERR_INVALID, "null argument");

@@ -192,7 +192,37 @@ struct dp_sim {
 
     ~dp_sim();  // leaves its batch first (batch.cpp)
 
+    // An overlapped readback (dp_sim_begin_read): the fields are copied to
+    // `read_stage` on the sim's stream, then to the caller's host buffer on
+    // `read_stream`, which waits for `read_staged`; `read_done` follows the
+    // host copy. One readback at a time.
+    DeviceBuffer read_stage;
+    hipStream_t read_stream = nullptr;
+    hipEvent_t read_staged = nullptr, read_done = nullptr;
+    bool read_pending = false;
+
+    // Waits for the readback in flight, if any (timeout 0: 120 s).
+    void finish_read(uint32_t timeout_ms) {
+        if (!read_pending) return;
+        const auto limit = std::chrono::milliseconds(timeout_ms ? timeout_ms : 120000);
+        const auto start = std::chrono::steady_clock::now();
+        for (;;) {
+            const hipError_t status = hipEventQuery(read_done);
+            if (status == hipSuccess) break;
+            if (status != hipErrorNotReady) check(status, "hipEventQuery");
+            if (std::chrono::steady_clock::now() - start > limit)
+                fail(DP_ERR_TIMEOUT, "the readback did not finish in time");
+            std::this_thread::sleep_for(std::chrono::microseconds(50));
+        }
+        read_pending = false;
+    }
+
     void release_stream() {
+        if (read_pending) (void)hipEventSynchronize(read_done);
+        read_pending = false;
+        if (read_staged) (void)hipEventDestroy(read_staged);
+        if (read_done) (void)hipEventDestroy(read_done);
+        if (read_stream) (void)hipStreamDestroy(read_stream);
         if (stream) (void)hipStreamSynchronize(stream);
         for (auto &ev : events)
             if (ev) (void)hipEventDestroy(ev);

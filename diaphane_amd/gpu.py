@@ -210,6 +210,20 @@ class Simulation:
         check(lib().dp_sim_read_magnetic(self._handle, _fptr(out), out.size))
         return out
 
+    def begin_read(self, which: int, out: np.ndarray) -> None:
+        """Starts copying E (BUFFER_ELECTRIC) or H (BUFFER_MAGNETIC) as of
+        this call into `out` and returns at once; later advances run while
+        the copy does. `out` (a PinnedBuffer's array for the copy to overlap)
+        must stay untouched until finish_read."""
+        out = self._field_out(out)
+        self._read_target = out  # kept alive while the copy may write it
+        check(lib().dp_sim_begin_read(self._handle, int(which), _fptr(out), out.size))
+
+    def finish_read(self, timeout_ms: int = 0) -> None:
+        """Waits for the readback begin_read started, if any."""
+        check(lib().dp_sim_finish_read(self._handle, int(timeout_ms)))
+        self._read_target = None
+
     def read_intensity(self) -> np.ndarray:
         out = np.empty(self.cells, dtype=np.float32)
         check(lib().dp_sim_read_intensity(self._handle, _fptr(out), out.size))

@@ -281,6 +281,19 @@ uint64_t dp_sim_accumulated_steps(const dp_sim *sim);
  * component-major, x fastest. Synchronous. */
 int dp_sim_read_electric(dp_sim *sim, float *out, size_t count);
 int dp_sim_read_magnetic(dp_sim *sim, float *out, size_t count);
+/* An overlapped read_electric / read_magnetic (no counterpart in the
+ * reference, whose readbacks block: src/gpu.rs:644-656). begin_read returns
+ * at once: E (which = DP_BUFFER_ELECTRIC) or H (DP_BUFFER_MAGNETIC) as of
+ * this call, after every step queued so far, is copied on the device to a
+ * staging buffer of the sim and from there to `out` (3·N floats) on a
+ * stream of its own, while later advances run. `out` must stay valid and
+ * untouched until finish_read returns; page-locked memory (dp_host_alloc)
+ * lets the host copy overlap the steps. One readback is in flight per sim:
+ * begin_read first waits for the previous one. finish_read waits for it
+ * (timeout 0: 120 s; DP_ERR_TIMEOUT) and returns at once when none is in
+ * flight. Destroying the sim waits for it too. */
+int dp_sim_begin_read(dp_sim *sim, int which, float *out, size_t count);
+int dp_sim_finish_read(dp_sim *sim, uint32_t timeout_ms);
 /* read_intensity (src/gpu.rs:702-705): N floats (the f64 sums rounded). */
 int dp_sim_read_intensity(dp_sim *sim, float *out, size_t count);
 int dp_sim_read_intensity_f64(dp_sim *sim, double *out, size_t count);
