@@ -285,7 +285,8 @@ int dp_sim_read_magnetic(dp_sim *sim, float *out, size_t count);
  * reference, whose readbacks block: src/gpu.rs:644-656). begin_read returns
  * at once: E (which = DP_BUFFER_ELECTRIC) or H (DP_BUFFER_MAGNETIC) as of
  * this call, after every step queued so far, is copied on the device to a
- * staging buffer of the sim and from there to `out` (3·N floats) on a
+ * staging buffer of the sim (12 bytes a cell of device memory, allocated at
+ * the first call and kept) and from there to `out` (3·N floats) on a
  * stream of its own, while later advances run. `out` must stay valid and
  * untouched until finish_read returns; page-locked memory (dp_host_alloc)
  * lets the host copy overlap the steps. One readback is in flight per sim:
