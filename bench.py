@@ -403,7 +403,7 @@ def run_served(context, requests: int, rate: float, chunk_steps: int, rows: int,
         "sequential": sequential,
         "speedup_vs_sequential": batched["value"] / sequential["value"],
         "kinds": {k: sum(1 for r in stream if r.kind == k) for k in workloads.SERVED_KINDS},
-        "chunk_cells": "every member" if chunk_cells < 0 else (chunk_cells or 64 << 20),
+        "chunk_cells": "every member" if chunk_cells < 0 else (chunk_cells or 128 << 20),
     }
 
 

@@ -320,10 +320,12 @@ int dp_server_create(dp_engine *engine, const dp_server_config *config, dp_serve
         if (c.chunk_steps == 0) c.chunk_steps = 32;
         if (c.host_threads == 0) c.host_threads = 4;
         if (c.host_threads > 64) fail(DP_ERR_INVALID, "at most 64 host threads");
-        // 64M cells a chunk: with two-step batches, 1.07x the throughput of
-        // 32M and a lower p99 (4.5 vs 6.1 s) on the 50 req/s stream, at a
-        // higher p50 (2.3 vs 1.4 s), measured on MI355X.
-        if (c.chunk_cells == 0) c.chunk_cells = 64ull << 20;
+        // 128M cells a chunk, measured on MI355X with two-step batches of 8
+        // rows over the 200-request stream: 83.8 G/s served at 50 req/s (p50
+        // 1.77 s, p99 3.95 s) and 90.8 at 200 req/s (4.13 s, 5.75 s), against
+        // 82.9 and 86.6 G/s at 64M (1.67 / 3.56 s and 3.18 / 5.87 s) and 78.0
+        // and 79.5 at 32M.
+        if (c.chunk_cells == 0) c.chunk_cells = 128ull << 20;
         server->config = c;
         check(hipSetDevice(engine->device), "hipSetDevice");
         for (hipEvent_t &e : server->marks) check(hipEventCreate(&e), "hipEventCreate");

@@ -42,7 +42,7 @@ class Server:
     in arrival order, through the plain single-sim path (the baseline).
     Otherwise resident requests share one batch; each chunk steps them in
     order of least remaining work until ``chunk_cells`` cells are stepping
-    (0: 64M; ``ALL_MEMBERS``: every member every chunk)."""
+    (0: 128M; ``ALL_MEMBERS``: every member every chunk)."""
 
     ALL_MEMBERS = (1 << 64) - 1
 

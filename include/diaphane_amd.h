@@ -552,7 +552,7 @@ typedef struct dp_server_config {
                                   left) until this many cells step in a
                                   chunk; the others wait that chunk, and
                                   none waits more than 64 chunks in a row.
-                                  0: 64M cells; UINT64_MAX: every member
+                                  0: 128M cells; UINT64_MAX: every member
                                   steps every chunk                       */
 } dp_server_config;
 enum { DP_REQUEST_QUEUED = 0, DP_REQUEST_PREPARING = 1, DP_REQUEST_READY = 2, DP_REQUEST_RUNNING = 3,
