@@ -235,8 +235,28 @@ def run_batched(context, steps: int, warmup: int, tile=None, members: str = "all
     }
 
 
+def batch_traffic(run: dict):
+    """HBM bytes per launch of the batch kernel on the 256 grids, from its
+    committed rocprofv3 counter summary when the run used the shape that
+    summary measured (two steps per launch, 8 rows, 32-plane slabs, all
+    members); None otherwise."""
+    tiling = run["tiling"]
+    if not (tiling.get("two_step") and not tiling.get("stack_waves") and tiling.get("rows") == 8
+            and tiling.get("depth") == 32 and len(run["instances"]) == 256):
+        return None, None
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
+                        "k_fused_pair_batch_batched_uneven_256.json")
+    if not os.path.exists(path):
+        return None, None
+    with open(path) as handle:
+        traffic = json.load(handle).get("traffic_bytes_per_launch")
+    return traffic, (f"profiles/{os.path.basename(path)}: rocprofv3 FETCH_SIZE x 2 + WRITE_SIZE per launch of this "
+                     f"kernel at this shape, from a separate counter run (not measured in this run)")
+
+
 def batched_summary(run: dict) -> dict:
     """The batched form's sub-object of the default benchmark line."""
+    traffic, source = batch_traffic(run)
     return {
         "workload": "batched_uneven_256",
         "metric": "aggregate Yee cell-updates/s, batched vs one at a time",
@@ -255,6 +275,8 @@ def batched_summary(run: dict) -> dict:
         "tiling": run["tiling"],
         "achieved_gbs": run["achieved_gbs"],
         "frac_of_hbm_peak": run["achieved_gbs"] / HBM_PEAK_GBS,
+        "traffic_bytes_per_launch": traffic,
+        "traffic_source": source,
     }
 
 
@@ -299,7 +321,8 @@ def bench_batched(args) -> int:
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": run["achieved_gbs"] / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": batch_traffic(run)[0],
+            "traffic_source": batch_traffic(run)[1],
             "kernel": run["kernel"],
             "bytes_per_cell": BYTES_PER_CELL,
             "launches": run["launches"],
