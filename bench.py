@@ -504,8 +504,9 @@ def main() -> int:
     parser.add_argument("--rows", type=int, default=0, help="fused-kernel tile rows (0 = default / autotune)")
     parser.add_argument("--depth", type=int, default=0, help="fused-kernel z-slab depth (0 = default)")
     parser.add_argument("--tile-flags", type=int, default=-1,
-                        help="tile flags (1 XCD remap, 2 non-temporal stores, 4 buffer addressing, 8 two steps "
-                             "per launch; batched: 16 one wave per tile; -1 = default)")
+                        help="tile flags (1 XCD remap, run length in bits 8..15; 2 non-temporal stores; 4 buffer "
+                             "addressing; 8 two steps per launch; 32 stacked waves, 2 or 4 in bits 16..19; "
+                             "batched: 16 one wave per tile; -1 = default)")
     parser.add_argument("--no-cpu", action="store_true", help="skip the CPU-oracle baseline")
     parser.add_argument("--no-batched", action="store_true",
                         help="default workload: skip the batched_uneven_256 comparison")
