@@ -9,14 +9,14 @@ materials are resident in HBM before the timed region starts.
 
 One step = one full leapfrog step of the whole grid (H, injections, E,
 injections, intensity). Prints one JSON line. The default line also carries
-``batched`` and ``served`` objects: the two workloads below, each against the
-same work run one solve at a time on the same GPU (``--no-batched``,
-``--no-served`` leave them out), and ``cpu_baseline`` times the C oracle on
-all the run's host threads and on one.
+``batched``, ``served`` (50 req/s) and ``served_200`` (200 req/s) objects: the
+two workloads below, each against the same work run one solve at a time on
+the same GPU (``--no-batched``, ``--no-served`` leave them out), and
+``cpu_baseline`` times the C oracle on all the run's host threads and on one.
 
 ``--workload batched_uneven_256``: 256 independent uneven solves (sides
 log-uniform in [48, 160] per axis, PEC or absorbing, seeded cavity modes),
-1000 steps each, advanced one launch per step for the whole batch; the same
+1000 steps each, advanced one launch per two steps for the whole batch; the same
 solves run one at a time on the same GPU are timed beside it, and a sample of
 members is checked bit for bit between the two.
 
