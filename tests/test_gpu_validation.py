@@ -1,13 +1,14 @@
-"""The reference's analytic acceptance tests (tests/validation.rs), run on the
-device itself.
+"""The reference's analytic acceptance tests (tests/validation.rs,
+tests/graded.rs), run on the device itself.
 
 The device is bit-identical to the oracle, which these same tests pin on the
 CPU (test_oracle_validation.py). Here they run directly against the engine,
 through the C ABI, at the reference's own tolerances: an exact discrete plane
 wave in four directions at two resolutions and in dielectrics (with the
 negative control that shows the check can fail), the plane-wave source's
-one-way launch, the PEC box's mirror symmetry, and the absorbing layer's
-reflection.
+one-way launch, the PEC box's mirror symmetry, the absorbing layer's
+reflection, and on graded grids a reversal round trip, agreement with a
+uniformly fine grid and the grading transition's reflection.
 """
 from __future__ import annotations
 
@@ -17,9 +18,10 @@ import numpy as np
 import pytest
 
 import oracle
-from diaphane_amd import Axis, Extent, Scene, Source, Waveform
+from diaphane_amd import Axis, Boundary, Extent, Grid, Scene, Source, Waveform
 from diaphane_amd.gpu import Simulation
-from test_oracle_validation import PlaneWave, pec_box
+from test_oracle_validation import (CELL, DOMAIN, FINE, TRANSVERSE, PlaneWave, band, pec_box, resample,
+                                    uniformly_fine)
 
 pytestmark = pytest.mark.gpu
 
@@ -112,8 +114,6 @@ def test_a_plane_wave_source_is_one_way_on_the_device(context):
 def test_a_conducting_box_cannot_tell_its_low_walls_from_its_high_ones_on_the_device(context):
     # tests/validation.rs:431-482
     n = 24
-    from diaphane_amd import Boundary
-
     scene = Scene.empty(Extent.cube(n), 1e-3).with_boundary(Boundary.PEC)
     frequency = scene.grid.frequency_for_resolution(12.0)
     scene.sources.append(Source.point([0.0] * 3, Axis.Z, Waveform.ricker(frequency)))
@@ -154,3 +154,73 @@ def test_the_absorbing_layer_reflects_below_the_stated_level_on_the_device(conte
     decibels = 20.0 * math.log10(reflected / incident)
     print(f"absorbing layer reflection on the device: {decibels:.1f} dB (bar -50 dB)", flush=True)
     assert decibels < -50.0
+
+
+# ---------------------------------------------------------------------------
+# Graded grids (tests/graded.rs), on the device
+# ---------------------------------------------------------------------------
+
+def device_trace(context, grid, steps) -> np.ndarray:
+    scene = Scene.on_grid(grid)
+    frequency = scene.grid.frequency_for_resolution(20.0)
+    scene.sources.append(Source.point([-0.012, 0.0, 0.0], Axis.Z, Waveform.ricker(frequency)))
+    px, py, pz = oracle.cell_containing(scene.grid, [-0.004, 0.0, 0.0])
+    sim = Simulation(context, scene)
+    nx, ny, _ = sim.extent
+    index = (pz * ny + py) * nx + px
+    out = []
+    for _ in range(steps):
+        sim.advance_by(1)
+        out.append(sim.component(sim.read_electric(), Axis.Z)[index])
+    sim.close()
+    return np.array(out, dtype=np.float32)
+
+
+def test_stepping_back_survives_a_graded_grid_on_the_device(context):
+    # tests/graded.rs:48-93
+    scene = Scene.cavity_on(Grid.graded([0.040, 0.028, 0.028], 1.0 / CELL, [band(0.004, FINE)]))
+    sim = Simulation(context, scene)
+    sim.advance_by(200)
+    marked = sim.read_electric()
+    excited = sim.energy().total()
+    sim.advance_by(150)
+    sim.reverse_by(150)
+    peak = float(np.abs(marked).max())
+    assert peak > 0.0
+    assert float(np.abs(marked - sim.read_electric()).max()) < 1e-4 * peak
+    sim.reverse_by(200)
+    assert sim.energy().total() < 1e-8 * excited
+    sim.close()
+
+
+def test_a_graded_grid_gives_the_same_answer_as_a_uniformly_fine_one_on_the_device(context):
+    # tests/graded.rs: the probe trace on a graded grid within 6% of the
+    # uniformly fine grid's, resampled to the graded time step
+    steps = 300
+    graded = Grid.graded([DOMAIN, TRANSVERSE, TRANSVERSE], 1.0 / CELL, [band(0.02, FINE)])
+    fine = uniformly_fine()
+    gs, fs = graded.time_step(), fine.time_step()
+    measured = device_trace(context, graded, steps)
+    expected = resample(device_trace(context, fine, int(np.float32(steps) * np.float32(gs) / np.float32(fs)) + 2),
+                        fs, len(measured), gs)
+    peak = float(np.abs(expected).max())
+    assert peak > 0.0
+    assert float(np.abs(measured - expected).max()) < 0.06 * peak
+
+
+def test_a_gentler_grading_transition_reflects_less_on_the_device(context):
+    # tests/graded.rs: the default cap of 1.15 reflects below −30 dB, a cap of
+    # 1.05 at least 3 dB less
+    reference = device_trace(context, uniformly_fine(), 340)
+    incident = float(np.abs(reference).max())
+    assert incident > 0.0
+
+    def reflection(max_ratio):
+        graded = Grid.graded_at([DOMAIN, TRANSVERSE, TRANSVERSE], 1.0 / CELL, [band(0.014, FINE)], max_ratio)
+        measured = device_trace(context, graded, 340)
+        return 20.0 * math.log10(float(np.abs(measured - reference).max()) / incident)
+
+    default, gentle = reflection(1.15), reflection(1.05)
+    print(f"grading transition on the device: {default:.1f} dB at a 1.15 cap, {gentle:.1f} dB at 1.05", flush=True)
+    assert gentle < default - 3.0
+    assert default < -30.0
