@@ -341,6 +341,18 @@ struct dp_batch {
                     best_depth = c.depth;
                     best_grid = c.grid;
                 }
+            // The winner's slab depth between the powers of two, as for a
+            // single sim.
+            const uint32_t winner_depth = best_depth;
+            for (uint32_t d : {winner_depth * 3 / 4, winner_depth * 3 / 2}) {
+                if (d < 2 || d == winner_depth) continue;
+                Candidate c{best_grid, best_rows, d, std::numeric_limits<float>::infinity()};
+                for (int round = 0; round < 2; ++round) c.ms = std::min(c.ms, time_candidate(c, 3 * trials));
+                if (c.ms < best) {
+                    best = c.ms;
+                    best_depth = d;
+                }
+            }
         }
         flags = (flags & ~tuned_bits) | best_grid;
         rows = best_rows;
