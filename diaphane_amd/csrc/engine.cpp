@@ -564,6 +564,9 @@ int dp_sim_set_tile(dp_sim *sim, int rows, int depth, int flags) {
         for (auto &t : sim->tiling) {
             t.rows = rows;
             t.depth = static_cast<uint32_t>(depth);
+            t.single_rows = 0;  // an odd last step takes the one-step default
+            t.single_depth = 0;
+            t.single_flags = 0;
             if (flags >= 0)
                 t.flags = flags & (DP_TILE_XCD_REMAP | DP_TILE_NONTEMPORAL | DP_TILE_BUFFER | DP_TILE_TWO_STEP |
                                    DP_TILE_STACK | 0xFF00 | 0xF0000);
@@ -659,6 +662,11 @@ int dp_sim_autotune(dp_sim *sim, uint32_t trials, int *rows, int *depth, int *fl
         // launches in two alternating rounds: neighbours in the sweep are
         // often within its noise.
         std::sort(sweep.begin(), sweep.end(), [](const Candidate &x, const Candidate &y) { return x.ms < y.ms; });
+        // The fastest one-step shape, for the odd last step of a two-step
+        // tiling.
+        const auto single = std::find_if(sweep.begin(), sweep.end(),
+                                         [](const Candidate &c) { return !(c.flags & DP_TILE_TWO_STEP); });
+        const Candidate one_step = single != sweep.end() ? *single : Candidate{0, 0, 0, 0.0f};
         sweep.resize(std::min<size_t>(sweep.size(), 4));
         for (Candidate &c : sweep) c.ms = std::numeric_limits<float>::infinity();
         for (int round = 0; round < 2; ++round)
@@ -689,6 +697,9 @@ int dp_sim_autotune(dp_sim *sim, uint32_t trials, int *rows, int *depth, int *fl
         tuned.rows = best_rows;
         tuned.depth = best_depth;
         tuned.flags = best_flags;
+        tuned.single_rows = one_step.rows;
+        tuned.single_depth = one_step.depth;
+        tuned.single_flags = one_step.flags;
         if (rows) *rows = best_rows;
         if (depth) *depth = static_cast<int>(best_depth);
         if (flags) *flags = best_flags;

@@ -172,7 +172,26 @@ struct dp_sim {
         int rows = 0;
         uint32_t depth = 0;
         int flags = DP_TILE_NONTEMPORAL;
+        // With DP_TILE_TWO_STEP, the one-step launch that takes an odd
+        // count's last step: rows 0 is the one-step default (4 rows, the size
+        // rule's depth, this tiling's store, addressing and XCD flags); the
+        // autotuner sets its fastest one-step shape here. The two-step rows
+        // do not carry over: 8 rows a wave ran the one-step kernel at 80³
+        // 2× slower than 4 with a plane-wave source.
+        int single_rows = 0;
+        uint32_t single_depth = 0;
+        int single_flags = 0;
     };
+    // The one-step shape of `t` (itself unless it takes two steps a launch).
+    static Tiling one_step_tiling(const Tiling &t) {
+        if (!(t.flags & DP_TILE_TWO_STEP)) return t;
+        Tiling one;
+        one.rows = t.single_rows ? t.single_rows : 4;
+        one.depth = t.single_rows ? t.single_depth : 0;
+        one.flags = t.single_rows ? t.single_flags
+                                  : t.flags & (DP_TILE_NONTEMPORAL | DP_TILE_BUFFER | DP_TILE_XCD_REMAP | 0xFF00);
+        return one;
+    }
     Tiling tiling[2];
     // Grids from this size start on stacked two-step tiles; below it one step
     // per launch (4 rows) is faster (measured on MI355X: 64³ 11.0 against
@@ -210,9 +229,16 @@ struct dp_sim {
             const hipError_t status = hipEventQuery(read_done);
             if (status == hipSuccess) break;
             if (status != hipErrorNotReady) check(status, "hipEventQuery");
-            if (std::chrono::steady_clock::now() - start > limit)
-                fail(DP_ERR_TIMEOUT, "the readback did not finish in time");
-            std::this_thread::sleep_for(std::chrono::microseconds(50));
+            const auto waited = std::chrono::steady_clock::now() - start;
+            if (waited > limit) fail(DP_ERR_TIMEOUT, "the readback did not finish in time");
+            // Polled without sleeping for the first 2 ms: a 50 µs sleep
+            // woke 60–100 µs late, and the next readback's staging copy,
+            // queued only after this returns, waited for it (80³ with a
+            // readback every 5 steps: about a third of each chunk).
+            if (waited < std::chrono::milliseconds(2))
+                std::this_thread::yield();
+            else
+                std::this_thread::sleep_for(std::chrono::microseconds(20));
         }
         read_pending = false;
     }
@@ -467,13 +493,15 @@ struct dp_sim {
                 a.n_magnetic = t.magnetic;
                 a.n_electric = t.electric;
                 a.intensity = accumulating ? intensity.as<double>() : nullptr;
-                const Tiling &t = active();
                 if (kernel == DP_KERNEL_FUSED) {
-                    fill_tiles(a, r, kAlignedX, t);
-                    launch_fused_aligned(a, r, (t.flags & DP_TILE_NONTEMPORAL) != 0, (t.flags & DP_TILE_BUFFER) != 0,
-                                         stream);
+                    // a two-step tiling's odd last step takes its one-step shape
+                    const Tiling tl = one_step_tiling(active());
+                    const int rows = pairs ? tl.rows : r;
+                    fill_tiles(a, rows, kAlignedX, tl);
+                    launch_fused_aligned(a, rows, (tl.flags & DP_TILE_NONTEMPORAL) != 0,
+                                         (tl.flags & DP_TILE_BUFFER) != 0, stream);
                 } else {
-                    fill_tiles(a, r, kOwnedX, t);
+                    fill_tiles(a, r, kOwnedX, active());
                     launch_fused(a, r, stream);
                 }
                 current = 1 - current;

@@ -291,8 +291,9 @@ int dp_sim_read_magnetic(dp_sim *sim, float *out, size_t count);
  * untouched until finish_read returns; page-locked memory (dp_host_alloc)
  * lets the host copy overlap the steps. One readback is in flight per sim:
  * begin_read first waits for the previous one. finish_read waits for it
- * (timeout 0: 120 s; DP_ERR_TIMEOUT) and returns at once when none is in
- * flight. Destroying the sim waits for it too. */
+ * (timeout 0: 120 s; DP_ERR_TIMEOUT), polling without sleeping for its first
+ * 2 ms, and returns at once when none is in flight. Destroying the sim waits
+ * for it too. */
 int dp_sim_begin_read(dp_sim *sim, int which, float *out, size_t count);
 int dp_sim_finish_read(dp_sim *sim, uint32_t timeout_ms);
 /* read_intensity (src/gpu.rs:702-705): N floats (the f64 sums rounded). */
@@ -389,7 +390,10 @@ int dp_sim_set_kernel(dp_sim *sim, int kernel);
  * of two at or below the number of x-y tiles / 50, within 1..32, or / 25
  * within 2..32 for two steps per launch: short slabs give a small grid
  * enough waves, long ones amortize a large grid's halo planes). flags < 0
- * keeps the current flags. Results never depend on any of it. */
+ * keeps the current flags. With DP_TILE_TWO_STEP, the last step of an odd
+ * count runs one step per launch on 4 rows with the size-aware depth and
+ * this tiling's store, addressing and XCD flags. Results never depend on any
+ * of it. */
 int dp_sim_set_tile(dp_sim *sim, int rows, int depth, int flags);
 /* Autotuner for DP_KERNEL_FUSED: times `trials` launches of every rows
  * (4, 6, 8 one step per launch with non-temporal stores; 6, 8, 10 two steps
@@ -397,8 +401,10 @@ int dp_sim_set_tile(dp_sim *sim, int rows, int depth, int flags);
  * stacked waves with cached stores and XCD runs of 8) × depth (1 to 64 in powers of two, up
  * to twice the grid's z extent) × addressing (pointer, buffer where it fits)
  * candidate on this sim's grid, times the four fastest again with 3·trials
- * launches in two alternating rounds, keeps the fastest per step and reports
- * it (any out pointer may be null). A sim keeps one tiling for plain steps and one
+ * launches in two alternating rounds, then three quarters and one and a half
+ * of the winner's depth the same way, keeps the fastest per step and reports
+ * it (any out pointer may be null). With a two-step winner it also keeps the
+ * sweep's fastest one-step shape for the last step of an odd count. A sim keeps one tiling for plain steps and one
  * for intensity-accumulating steps; the autotuner sets the one for the
  * sim's current mode (dp_sim_set_tile sets both). The state is unchanged:
  * trial launches read the current fields, write only the idle half of each
