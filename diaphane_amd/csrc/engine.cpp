@@ -564,9 +564,6 @@ int dp_sim_set_tile(dp_sim *sim, int rows, int depth, int flags) {
         for (auto &t : sim->tiling) {
             t.rows = rows;
             t.depth = static_cast<uint32_t>(depth);
-            t.single_rows = 0;  // an odd last step takes the one-step default
-            t.single_depth = 0;
-            t.single_flags = 0;
             if (flags >= 0)
                 t.flags = flags & (DP_TILE_XCD_REMAP | DP_TILE_NONTEMPORAL | DP_TILE_BUFFER | DP_TILE_TWO_STEP |
                                    DP_TILE_STACK | 0xFF00 | 0xF0000);
@@ -588,12 +585,23 @@ int dp_sim_autotune(dp_sim *sim, uint32_t trials, int *rows, int *depth, int *fl
         const bool buffer_fits = buffer_addressing_fits(sim->cells, sim->grid.extent[0], sim->grid.extent[1]);
         const int flag_choices[] = {DP_TILE_NONTEMPORAL, DP_TILE_NONTEMPORAL | DP_TILE_BUFFER, 0, DP_TILE_BUFFER};
         // Trial launches read the current half of each ping-pong pair and
-        // write the idle half, with no injections; a sim that accumulates
-        // intensity is timed with the read-modify-write going to a scratch
-        // buffer. The state is unchanged.
+        // write the idle half, with the injections of the sim's next two
+        // steps (a source's walk between the half-steps costs a small grid's
+        // few waves much of their step time, and changes which shape is
+        // fastest); a sim that accumulates intensity is timed with the
+        // read-modify-write going to a scratch buffer. The state is unchanged.
+        std::vector<InjDesc> trial_injections;
+        const auto table = sim->build_injections(sim->step, 2, +1, trial_injections);
+        const InjDesc *device_inj = nullptr;
+        struct Release {
+            dp_sim *sim;
+            dp_sim::InjectionSlot *slot;
+            ~Release() { sim->release_injections(slot); }
+        } release{sim, sim->upload_injections(trial_injections, device_inj)};
         StepArgs a = sim->args(sim->current, 1 - sim->current);
-        a.injections = sim->empty_injections.as<InjDesc>();
-        a.n_magnetic = a.n_electric = 0;
+        a.injections = device_inj + table[0].offset;
+        a.n_magnetic = table[0].magnetic;
+        a.n_electric = table[0].electric;
         DeviceBuffer scratch;
         a.intensity = nullptr;
         if (sim->accumulating) {
@@ -605,8 +613,9 @@ int dp_sim_autotune(dp_sim *sim, uint32_t trials, int *rows, int *depth, int *fl
         float best = std::numeric_limits<float>::infinity();
         int best_rows = 4, best_flags = DP_TILE_NONTEMPORAL;
         uint32_t best_depth = 32;
-        a.injections2 = a.injections;
-        a.n_magnetic2 = a.n_electric2 = 0;
+        a.injections2 = device_inj + table[1].offset;
+        a.n_magnetic2 = table[1].magnetic;
+        a.n_electric2 = table[1].electric;
         // Milliseconds per step of one shape over `launches` timed launches.
         auto time_shape = [&](int r, uint32_t d, int f, uint32_t launches) {
             const bool two_step = (f & DP_TILE_TWO_STEP) != 0;
@@ -662,11 +671,6 @@ int dp_sim_autotune(dp_sim *sim, uint32_t trials, int *rows, int *depth, int *fl
         // launches in two alternating rounds: neighbours in the sweep are
         // often within its noise.
         std::sort(sweep.begin(), sweep.end(), [](const Candidate &x, const Candidate &y) { return x.ms < y.ms; });
-        // The fastest one-step shape, for the odd last step of a two-step
-        // tiling.
-        const auto single = std::find_if(sweep.begin(), sweep.end(),
-                                         [](const Candidate &c) { return !(c.flags & DP_TILE_TWO_STEP); });
-        const Candidate one_step = single != sweep.end() ? *single : Candidate{0, 0, 0, 0.0f};
         sweep.resize(std::min<size_t>(sweep.size(), 4));
         for (Candidate &c : sweep) c.ms = std::numeric_limits<float>::infinity();
         for (int round = 0; round < 2; ++round)
@@ -697,9 +701,6 @@ int dp_sim_autotune(dp_sim *sim, uint32_t trials, int *rows, int *depth, int *fl
         tuned.rows = best_rows;
         tuned.depth = best_depth;
         tuned.flags = best_flags;
-        tuned.single_rows = one_step.rows;
-        tuned.single_depth = one_step.depth;
-        tuned.single_flags = one_step.flags;
         if (rows) *rows = best_rows;
         if (depth) *depth = static_cast<int>(best_depth);
         if (flags) *flags = best_flags;

@@ -172,24 +172,21 @@ struct dp_sim {
         int rows = 0;
         uint32_t depth = 0;
         int flags = DP_TILE_NONTEMPORAL;
-        // With DP_TILE_TWO_STEP, the one-step launch that takes an odd
-        // count's last step: rows 0 is the one-step default (4 rows, the size
-        // rule's depth, this tiling's store, addressing and XCD flags); the
-        // autotuner sets its fastest one-step shape here. The two-step rows
-        // do not carry over: 8 rows a wave ran the one-step kernel at 80³
-        // 2× slower than 4 with a plane-wave source.
-        int single_rows = 0;
-        uint32_t single_depth = 0;
-        int single_flags = 0;
     };
-    // The one-step shape of `t` (itself unless it takes two steps a launch).
+    // The one-step shape of `t` (itself unless it takes two steps a launch):
+    // with DP_TILE_TWO_STEP, the launch that takes an odd count's last step
+    // runs 4 rows with the size rule's depth and `t`'s store, addressing and
+    // XCD flags. The two-step rows do not carry over: 8 rows a wave ran the
+    // one-step kernel at 80³ 3× slower than 4 with a plane-wave source (93
+    // against 29 µs). The autotuner's fastest one-step shape, taken from its
+    // one short sweep, was not kept for it: its pick varied from run to run
+    // and once cost the 80³ readback job 40%.
     static Tiling one_step_tiling(const Tiling &t) {
         if (!(t.flags & DP_TILE_TWO_STEP)) return t;
         Tiling one;
-        one.rows = t.single_rows ? t.single_rows : 4;
-        one.depth = t.single_rows ? t.single_depth : 0;
-        one.flags = t.single_rows ? t.single_flags
-                                  : t.flags & (DP_TILE_NONTEMPORAL | DP_TILE_BUFFER | DP_TILE_XCD_REMAP | 0xFF00);
+        one.rows = 4;
+        one.depth = 0;
+        one.flags = t.flags & (DP_TILE_NONTEMPORAL | DP_TILE_BUFFER | DP_TILE_XCD_REMAP | 0xFF00);
         return one;
     }
     Tiling tiling[2];

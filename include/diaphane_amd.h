@@ -403,12 +403,12 @@ int dp_sim_set_tile(dp_sim *sim, int rows, int depth, int flags);
  * candidate on this sim's grid, times the four fastest again with 3·trials
  * launches in two alternating rounds, then three quarters and one and a half
  * of the winner's depth the same way, keeps the fastest per step and reports
- * it (any out pointer may be null). With a two-step winner it also keeps the
- * sweep's fastest one-step shape for the last step of an odd count. A sim keeps one tiling for plain steps and one
+ * it (any out pointer may be null). A sim keeps one tiling for plain steps and one
  * for intensity-accumulating steps; the autotuner sets the one for the
  * sim's current mode (dp_sim_set_tile sets both). The state is unchanged:
- * trial launches read the current fields, write only the idle half of each
- * ping-pong pair, and accumulate into a scratch buffer. Synchronous. */
+ * trial launches read the current fields, apply the injections of the sim's
+ * next two steps, write only the idle half of each ping-pong pair, and
+ * accumulate into a scratch buffer. Synchronous. */
 int dp_sim_autotune(dp_sim *sim, uint32_t trials, int *rows, int *depth, int *flags, float *ms_per_step);
 
 /* HIP-event timing on the sim's stream: record event `slot` (0..7) now;
