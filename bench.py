@@ -581,14 +581,19 @@ def main() -> int:
     phase_a = accumulate_from if accumulate_from is not None else steps
     phase_b = steps - phase_a
 
-    # The application's readback buffer: page-locked host memory from the
-    # engine, reused for every readback.
-    host_e, pinned = None, None
+    # The application's readback buffers: two page-locked host buffers from
+    # the engine, used in turn (a viewer shows one frame while the next one
+    # lands).
+    frames = []
     if readback:
         from diaphane_amd.gpu import PinnedBuffer
 
-        pinned = PinnedBuffer(3 * cells)
-        host_e = pinned.array
+        frames = [PinnedBuffer(3 * cells), PinnedBuffer(3 * cells)]
+        # Setup, like the warm-up steps: the sim's two staging buffers are
+        # allocated at their first readback and the host buffers touched.
+        for frame in frames:
+            sim.begin_read(BUFFER_ELECTRIC, frame.array)
+        sim.finish_read()
 
     def run(count: int) -> None:
         if not readback:
@@ -599,9 +604,10 @@ def main() -> int:
             chunk = min(readback, count - done)
             sim.advance_by(chunk)
             # This chunk's E to the host, overlapped with the next chunk's
-            # steps; the previous readback completes first.
-            sim.finish_read()
-            sim.begin_read(BUFFER_ELECTRIC, host_e)
+            # steps; once it is queued, the previous chunk's E has landed in
+            # the other buffer before the next chunk is queued.
+            sim.begin_read(BUFFER_ELECTRIC, frames[(done // readback) % 2].array)
+            sim.wait_reads(1)
             done += chunk
 
     launches = {"plain": 0, "with_intensity": 0}
@@ -679,8 +685,10 @@ def main() -> int:
             "extent": list(sim.extent),
             "steps_with_intensity": phase_b,
             "readback_every": readback,
-            "readback": ("E of each chunk into page-locked memory by dp_sim_begin_read, overlapped with the "
-                         "next chunk's steps; all land inside the timed region") if readback else None,
+            "readback": ("E of each chunk into one of two page-locked buffers by dp_sim_begin_read, overlapped "
+                         "with the next chunk's steps; each chunk waits (dp_sim_wait_reads) for the previous "
+                         "chunk's E to land before queueing the next; all land inside the timed region")
+            if readback else None,
             "tiling": tiling,
             "note": work.note,
         },

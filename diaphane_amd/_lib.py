@@ -93,6 +93,7 @@ def _declare(lib: ctypes.CDLL) -> None:
         "dp_sim_read_electric": (c_int, [sim, POINTER(c_float), c_size_t]),
         "dp_sim_read_magnetic": (c_int, [sim, POINTER(c_float), c_size_t]),
         "dp_sim_begin_read": (c_int, [sim, c_int, POINTER(c_float), c_size_t]),
+        "dp_sim_wait_reads": (c_int, [sim, c_uint32, c_uint32]),
         "dp_sim_finish_read": (c_int, [sim, c_uint32]),
         "dp_sim_read_intensity": (c_int, [sim, POINTER(c_float), c_size_t]),
         "dp_sim_read_intensity_f64": (c_int, [sim, POINTER(c_double), c_size_t]),

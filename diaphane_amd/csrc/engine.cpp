@@ -416,32 +416,40 @@ int dp_sim_begin_read(dp_sim *sim, int which, float *out, size_t count) {
         if (which != DP_BUFFER_ELECTRIC && which != DP_BUFFER_MAGNETIC)
             fail(DP_ERR_INVALID, "which must be DP_BUFFER_ELECTRIC or DP_BUFFER_MAGNETIC");
         if (count != 3 * sim->cells) fail(DP_ERR_SIZE, "buffer length must be 3 x cell count");
-        sim->finish_read(0);  // the staging buffer holds one readback at a time
-        if (!sim->read_stream) {
+        sim->wait_reads(dp_sim::kReadSlots - 1, 0);  // the slot this one takes is free
+        if (!sim->read_stream)
             check(hipStreamCreateWithFlags(&sim->read_stream, hipStreamNonBlocking), "hipStreamCreate");
-            check(hipEventCreateWithFlags(&sim->read_staged, hipEventDisableTiming), "hipEventCreate");
-            check(hipEventCreateWithFlags(&sim->read_done, hipEventDisableTiming), "hipEventCreate");
-        }
+        dp_sim::ReadSlot &slot = sim->read_slots[sim->next_read_slot];
+        if (!slot.staged) check(hipEventCreateWithFlags(&slot.staged, hipEventDisableTiming), "hipEventCreate");
+        if (!slot.done) check(hipEventCreateWithFlags(&slot.done, hipEventDisableTiming), "hipEventCreate");
         const size_t bytes = count * sizeof(float);
-        if (sim->read_stage.bytes < bytes) sim->read_stage.allocate(bytes);
+        if (slot.stage.bytes < bytes) slot.stage.allocate(bytes);
         const DeviceBuffer &field = (which == DP_BUFFER_ELECTRIC ? sim->e : sim->h)[sim->current];
         // On the sim's stream, after every step queued so far: the fields as
-        // of this call. The host copy then runs beside the steps that follow.
-        check(hipMemcpyAsync(sim->read_stage.ptr, field.ptr, bytes, hipMemcpyDeviceToDevice, sim->stream),
+        // of this call. The host copy then runs beside the steps that follow,
+        // after the host copies queued before it.
+        check(hipMemcpyAsync(slot.stage.ptr, field.ptr, bytes, hipMemcpyDeviceToDevice, sim->stream),
               "hipMemcpyAsync");
-        check(hipEventRecord(sim->read_staged, sim->stream), "hipEventRecord");
-        check(hipStreamWaitEvent(sim->read_stream, sim->read_staged, 0), "hipStreamWaitEvent");
-        check(hipMemcpyAsync(out, sim->read_stage.ptr, bytes, hipMemcpyDeviceToHost, sim->read_stream),
-              "hipMemcpyAsync");
-        check(hipEventRecord(sim->read_done, sim->read_stream), "hipEventRecord");
-        sim->read_pending = true;
+        check(hipEventRecord(slot.staged, sim->stream), "hipEventRecord");
+        check(hipStreamWaitEvent(sim->read_stream, slot.staged, 0), "hipStreamWaitEvent");
+        check(hipMemcpyAsync(out, slot.stage.ptr, bytes, hipMemcpyDeviceToHost, sim->read_stream), "hipMemcpyAsync");
+        check(hipEventRecord(slot.done, sim->read_stream), "hipEventRecord");
+        slot.pending = true;
+        sim->next_read_slot = (sim->next_read_slot + 1) % dp_sim::kReadSlots;
+    });
+}
+
+int dp_sim_wait_reads(dp_sim *sim, uint32_t in_flight, uint32_t timeout_ms) {
+    return guarded([&] {
+        if (!sim) fail(DP_ERR_INVALID, "null sim");
+        sim->wait_reads(int(std::min<uint32_t>(in_flight, dp_sim::kReadSlots)), timeout_ms);
     });
 }
 
 int dp_sim_finish_read(dp_sim *sim, uint32_t timeout_ms) {
     return guarded([&] {
         if (!sim) fail(DP_ERR_INVALID, "null sim");
-        sim->finish_read(timeout_ms);
+        sim->wait_reads(0, timeout_ms);
     });
 }
 

@@ -208,43 +208,61 @@ struct dp_sim {
 
     ~dp_sim();  // leaves its batch first (batch.cpp)
 
-    // An overlapped readback (dp_sim_begin_read): the fields are copied to
-    // `read_stage` on the sim's stream, then to the caller's host buffer on
-    // `read_stream`, which waits for `read_staged`; `read_done` follows the
-    // host copy. One readback at a time.
-    DeviceBuffer read_stage;
+    // Overlapped readbacks (dp_sim_begin_read): the fields are copied to a
+    // slot's `stage` on the sim's stream, then to the caller's host buffer on
+    // `read_stream`, which waits for the slot's `staged`; `done` follows the
+    // host copy. Two slots, used in turn, so a readback can be queued while
+    // the previous one is still on its way to the host; the host copies run
+    // in order on the one stream.
+    struct ReadSlot {
+        DeviceBuffer stage;
+        hipEvent_t staged = nullptr, done = nullptr;
+        bool pending = false;
+    };
+    static constexpr int kReadSlots = 2;
+    ReadSlot read_slots[kReadSlots];
+    int next_read_slot = 0;  // the slot the next readback takes
     hipStream_t read_stream = nullptr;
-    hipEvent_t read_staged = nullptr, read_done = nullptr;
-    bool read_pending = false;
 
-    // Waits for the readback in flight, if any (timeout 0: 120 s).
-    void finish_read(uint32_t timeout_ms) {
-        if (!read_pending) return;
+    int reads_in_flight() const {
+        int n = 0;
+        for (const ReadSlot &r : read_slots) n += r.pending ? 1 : 0;
+        return n;
+    }
+
+    // Waits until at most `keep` readbacks are in flight, the oldest first
+    // (timeout 0: 120 s).
+    void wait_reads(int keep, uint32_t timeout_ms) {
         const auto limit = std::chrono::milliseconds(timeout_ms ? timeout_ms : 120000);
         const auto start = std::chrono::steady_clock::now();
-        for (;;) {
-            const hipError_t status = hipEventQuery(read_done);
-            if (status == hipSuccess) break;
-            if (status != hipErrorNotReady) check(status, "hipEventQuery");
-            const auto waited = std::chrono::steady_clock::now() - start;
-            if (waited > limit) fail(DP_ERR_TIMEOUT, "the readback did not finish in time");
-            // Polled without sleeping for the first 2 ms: a 50 µs sleep
-            // woke 60–100 µs late, and the next readback's staging copy,
-            // queued only after this returns, waited for it (80³ with a
-            // readback every 5 steps: about a third of each chunk).
-            if (waited < std::chrono::milliseconds(2))
-                std::this_thread::yield();
-            else
-                std::this_thread::sleep_for(std::chrono::microseconds(20));
+        while (reads_in_flight() > keep) {
+            // the older of two in flight is the slot the next readback takes
+            ReadSlot &r = read_slots[read_slots[next_read_slot].pending ? next_read_slot : 1 - next_read_slot];
+            for (;;) {
+                const hipError_t status = hipEventQuery(r.done);
+                if (status == hipSuccess) break;
+                if (status != hipErrorNotReady) check(status, "hipEventQuery");
+                const auto waited = std::chrono::steady_clock::now() - start;
+                if (waited > limit) fail(DP_ERR_TIMEOUT, "the readback did not finish in time");
+                // Polled without sleeping for the first 2 ms: a 50 µs sleep
+                // woke 60–100 µs late, and a caller that queues the next
+                // steps after this returns left the device idle meanwhile.
+                if (waited < std::chrono::milliseconds(2))
+                    std::this_thread::yield();
+                else
+                    std::this_thread::sleep_for(std::chrono::microseconds(20));
+            }
+            r.pending = false;
         }
-        read_pending = false;
     }
 
     void release_stream() {
-        if (read_pending) (void)hipEventSynchronize(read_done);
-        read_pending = false;
-        if (read_staged) (void)hipEventDestroy(read_staged);
-        if (read_done) (void)hipEventDestroy(read_done);
+        for (ReadSlot &r : read_slots) {
+            if (r.pending) (void)hipEventSynchronize(r.done);
+            r.pending = false;
+            if (r.staged) (void)hipEventDestroy(r.staged);
+            if (r.done) (void)hipEventDestroy(r.done);
+        }
         if (read_stream) (void)hipStreamDestroy(read_stream);
         if (stream) (void)hipStreamSynchronize(stream);
         for (auto &ev : events)

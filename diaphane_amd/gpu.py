@@ -114,6 +114,8 @@ class Simulation:
 
     TIMEOUT_MS = 120_000
 
+    _read_targets: tuple = ()  # host buffers of the readbacks in flight
+
     def __init__(self, context: Context, scene: Scene):
         self._context = context
         self._scene_c = scene.to_c()
@@ -213,16 +215,26 @@ class Simulation:
     def begin_read(self, which: int, out: np.ndarray) -> None:
         """Starts copying E (BUFFER_ELECTRIC) or H (BUFFER_MAGNETIC) as of
         this call into `out` and returns at once; later advances run while
-        the copy does. `out` (a PinnedBuffer's array for the copy to overlap)
-        must stay untouched until finish_read."""
+        the copy does. Two readbacks can be in flight (with two in flight
+        this first waits for the older). `out` (a PinnedBuffer's array for
+        the copy to overlap) must stay untouched until its readback has
+        landed: wait_reads(1) after the next begin_read, or finish_read."""
         out = self._field_out(out)
-        self._read_target = out  # kept alive while the copy may write it
         check(lib().dp_sim_begin_read(self._handle, int(which), _fptr(out), out.size))
+        # kept alive while the copies may write them
+        self._read_targets = (self._read_targets + (out,))[-2:]
+
+    def wait_reads(self, in_flight: int, timeout_ms: int = 0) -> None:
+        """Waits until at most `in_flight` readbacks are in flight, the
+        oldest landing first."""
+        check(lib().dp_sim_wait_reads(self._handle, int(in_flight), int(timeout_ms)))
+        keep = min(int(in_flight), len(self._read_targets))
+        self._read_targets = self._read_targets[len(self._read_targets) - keep:]
 
     def finish_read(self, timeout_ms: int = 0) -> None:
-        """Waits for the readback begin_read started, if any."""
+        """Waits for every readback in flight."""
         check(lib().dp_sim_finish_read(self._handle, int(timeout_ms)))
-        self._read_target = None
+        self._read_targets = ()
 
     def read_intensity(self) -> np.ndarray:
         out = np.empty(self.cells, dtype=np.float32)

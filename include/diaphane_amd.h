@@ -285,16 +285,21 @@ int dp_sim_read_magnetic(dp_sim *sim, float *out, size_t count);
  * reference, whose readbacks block: src/gpu.rs:644-656). begin_read returns
  * at once: E (which = DP_BUFFER_ELECTRIC) or H (DP_BUFFER_MAGNETIC) as of
  * this call, after every step queued so far, is copied on the device to a
- * staging buffer of the sim (12 bytes a cell of device memory, allocated at
- * the first call and kept) and from there to `out` (3·N floats) on a
- * stream of its own, while later advances run. `out` must stay valid and
- * untouched until finish_read returns; page-locked memory (dp_host_alloc)
- * lets the host copy overlap the steps. One readback is in flight per sim:
- * begin_read first waits for the previous one. finish_read waits for it
- * (timeout 0: 120 s; DP_ERR_TIMEOUT), polling without sleeping for its first
- * 2 ms, and returns at once when none is in flight. Destroying the sim waits
- * for it too. */
+ * staging buffer of the sim and from there to `out` (3·N floats) on a
+ * stream of its own, while later advances run. Two readbacks can be in
+ * flight per sim, each with its own staging buffer (12 bytes a cell of
+ * device memory each, allocated at first use and kept); their host copies
+ * land in the order they were begun, and begin_read with two in flight
+ * first waits for the older. `out` must stay valid and untouched until its
+ * readback has landed; page-locked memory (dp_host_alloc) lets the host copy
+ * overlap the steps. wait_reads waits until at most `in_flight` readbacks
+ * are in flight (in_flight 1 after a begin_read: every earlier one has
+ * landed, so a caller can use the previous buffer of two while the newest
+ * copy runs); finish_read waits for all of them (wait_reads with 0). Both
+ * poll without sleeping for their first 2 ms (timeout 0: 120 s;
+ * DP_ERR_TIMEOUT). Destroying the sim waits for every readback in flight. */
 int dp_sim_begin_read(dp_sim *sim, int which, float *out, size_t count);
+int dp_sim_wait_reads(dp_sim *sim, uint32_t in_flight, uint32_t timeout_ms);
 int dp_sim_finish_read(dp_sim *sim, uint32_t timeout_ms);
 /* read_intensity (src/gpu.rs:702-705): N floats (the f64 sums rounded). */
 int dp_sim_read_intensity(dp_sim *sim, float *out, size_t count);

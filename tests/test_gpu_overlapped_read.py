@@ -1,11 +1,13 @@
-"""Overlapped readbacks (dp_sim_begin_read / dp_sim_finish_read).
+"""Overlapped readbacks (dp_sim_begin_read / dp_sim_wait_reads /
+dp_sim_finish_read).
 
 begin_read captures E or H as of the call, after every step queued before
 it, and copies it to the host while later advances run. The host buffer
 must then hold exactly the oracle's fields at that step, bit for bit, for
 one and two steps per launch (where the launches after the call overwrite
-the buffer the fields were read from), for a batch member while the batch
-keeps stepping, and when the sim is destroyed with the copy in flight.
+the buffer the fields were read from), with two readbacks in flight and a
+third waiting for the older, for a batch member while the batch keeps
+stepping, and when the sim is destroyed with the copy in flight.
 """
 from __future__ import annotations
 
@@ -50,7 +52,7 @@ def test_readbacks_hold_the_fields_of_their_step(context, tile):
     sim.finish_read()
     e46, h46 = oracle_at(scene, 46)
     assert same_bits(h_buf.array, h46)
-    # a second readback begun while one is in flight waits for the first
+    # a second readback begun while one is in flight runs behind it
     sim.begin_read(BUFFER_ELECTRIC, e_buf.array)
     sim.advance_by(3)
     sim.begin_read(BUFFER_MAGNETIC, h_buf.array)
@@ -59,6 +61,53 @@ def test_readbacks_hold_the_fields_of_their_step(context, tile):
     _, h49 = oracle_at(scene, 49)
     assert same_bits(h_buf.array, h49)
     sim.finish_read()  # nothing in flight: returns at once
+    sim.close()
+
+
+@pytest.mark.parametrize("tile", [None, (8, 2, TILE_TWO_STEP), (4, 2, TILE_TWO_STEP | TILE_STACK | TILE_BUFFER)],
+                         ids=["default", "two-step", "stacked"])
+def test_two_readbacks_in_flight_land_in_order(context, tile):
+    scene = Scene.photon(Extent(70, 36, 30))
+    sim = Simulation(context, scene)
+    if tile:
+        sim.set_tile(*tile)
+    bufs = [PinnedBuffer(3 * sim.cells) for _ in range(3)]
+    sim.advance_by(11)
+    sim.begin_read(BUFFER_ELECTRIC, bufs[0].array)
+    sim.advance_by(6)
+    sim.begin_read(BUFFER_MAGNETIC, bufs[1].array)
+    sim.wait_reads(1)  # the older of the two has landed
+    e11, _ = oracle_at(scene, 11)
+    assert float(np.abs(e11).max()) > 0.0
+    assert same_bits(bufs[0].array, e11)
+    sim.advance_by(3)
+    sim.begin_read(BUFFER_ELECTRIC, bufs[2].array)  # takes the slot the first one freed
+    sim.advance_by(5)
+    sim.finish_read()
+    _, h17 = oracle_at(scene, 17)
+    e20, _ = oracle_at(scene, 20)
+    assert same_bits(bufs[1].array, h17)
+    assert same_bits(bufs[2].array, e20)
+    sim.wait_reads(5)  # nothing in flight: returns at once
+    sim.close()
+
+
+def test_a_readback_begun_with_two_in_flight_waits_for_the_older(context):
+    scene = Scene.photon(Extent(52, 40, 34))
+    sim = Simulation(context, scene)
+    bufs = [PinnedBuffer(3 * sim.cells) for _ in range(5)]
+    done = 0
+    for k, buf in enumerate(bufs):
+        sim.advance_by(4 + k)
+        done += 4 + k
+        sim.begin_read(BUFFER_ELECTRIC, buf.array)  # from the third on, waits for the oldest
+    sim.advance_by(3)
+    sim.finish_read()
+    done = 0
+    for k, buf in enumerate(bufs):
+        done += 4 + k
+        expected, _ = oracle_at(scene, done)
+        assert same_bits(buf.array, expected), f"readback {k} at step {done}"
     sim.close()
 
 
