@@ -570,7 +570,16 @@ def main() -> int:
     tile_gbs = sim.tile_probe(tiling["rows"] if tiling["rows"] in (4, 6, 8) else 0, tiling["depth"], 5)
 
     if args.warmup:
-        sim.advance_by(args.warmup)
+        # The warm-up steps take the timed sequence's phases (and allocate the
+        # intensity sum's buffer where it has one, as its first accumulating
+        # step would).
+        if accumulate_from is not None and accumulate_from < steps:
+            sim.advance_by(args.warmup - args.warmup // 2)
+            sim.set_accumulate_intensity(True)
+            sim.advance_by(args.warmup // 2)
+            sim.set_accumulate_intensity(False)
+        else:
+            sim.advance_by(args.warmup)
         sim.wait()
         sim.reset()
         sim.wait()
