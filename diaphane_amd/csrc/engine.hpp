@@ -158,6 +158,27 @@ struct dp_sim {
     static constexpr int kInjectionSlots = 4;
     InjectionSlot injection_slots[kInjectionSlots];
     int next_injection_slot = 0;
+    // One step's injections in a table: magnetic ones from `offset`, then
+    // the electric ones.
+    struct StepInjections {
+        uint32_t offset, magnetic, electric;
+    };
+    // The forward injection tables of steps first + 1 to first + steps,
+    // uploaded once to a slot and reused by every advance they cover: a
+    // step's injections depend only on its number (the sources are fixed at
+    // creation). An advance of a few steps, as a viewer makes each frame,
+    // otherwise built its table on the host and copied it to the device on
+    // every call. A window is never written after its upload; the next one
+    // takes another slot.
+    struct InjectionWindow {
+        bool valid = false;
+        uint64_t first = 0, steps = 0;
+        std::vector<StepInjections> table;
+        std::vector<InjDesc> all;         // host copy (the two-pass path launches from it)
+        const InjDesc *device = nullptr;  // its device table
+        InjectionSlot *slot = nullptr;    // the slot holding it (nullptr: no injections)
+    } window;
+    static constexpr uint64_t kInjectionWindowSteps = 64;
     int current = 0;  // which of the ping-pong pair holds the state
 
     uint64_t step = 0;
@@ -322,9 +343,6 @@ struct dp_sim {
     // Host-side injection tables for `steps` steps starting after `first`
     // (src/gpu.rs:442-465): per step the magnetic ones, then the electric
     // ones, each sign-scaled, zero values dropped like the reference does.
-    struct StepInjections {
-        uint32_t offset, magnetic, electric;
-    };
     std::vector<StepInjections> build_injections(uint64_t first, uint64_t steps, int direction,
                                                  std::vector<InjDesc> &all) {
         std::vector<StepInjections> table(steps);
@@ -465,7 +483,7 @@ struct dp_sim {
 
     // One launch of the two-step kernel: steps `first` and `first + 1` of
     // `table`.
-    void launch_pair(const std::vector<StepInjections> &table, uint64_t first, const InjDesc *device_inj) {
+    void launch_pair(const StepInjections *table, uint64_t first, const InjDesc *device_inj) {
         const Tiling &tl = active();
         StepArgs a = args(current, 1 - current);
         a.injections = device_inj + table[first].offset;
@@ -481,14 +499,23 @@ struct dp_sim {
 
     void advance(uint64_t steps) {
         if (steps == 0) return;
-        std::vector<InjDesc> all;
-        auto table = build_injections(step, steps, +1, all);
-        const InjDesc *device_inj = nullptr;
+        if (!window.valid || step < window.first || step + steps > window.first + window.steps) {
+            window.valid = false;
+            const uint64_t n = std::max<uint64_t>(steps, kInjectionWindowSteps);
+            window.table = build_injections(step, n, +1, window.all);
+            window.slot = upload_injections(window.all, window.device);
+            window.first = step;
+            window.steps = n;
+            window.valid = true;
+        }
+        const StepInjections *table = window.table.data() + (step - window.first);
+        const std::vector<InjDesc> &all = window.all;
+        const InjDesc *device_inj = window.device;
         struct Release {
             dp_sim *sim;
             InjectionSlot *slot;
             ~Release() { sim->release_injections(slot); }
-        } release{this, upload_injections(all, device_inj)};
+        } release{this, window.slot};
         if (accumulating) ensure_intensity();
         const int r = effective_rows();
         const bool pairs = kernel == DP_KERNEL_FUSED && (active().flags & DP_TILE_TWO_STEP);
