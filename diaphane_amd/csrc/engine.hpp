@@ -169,7 +169,10 @@ struct dp_sim {
     // creation). An advance of a few steps, as a viewer makes each frame,
     // otherwise built its table on the host and copied it to the device on
     // every call. A window is never written after its upload; the next one
-    // takes another slot.
+    // takes another slot. Its slot's event is recorded once, when the window
+    // is retired, after every launch queued with it (one record per call put
+    // a marker between the launches of one-step calls: 15.5 against 12.9
+    // µs a step at 48³).
     struct InjectionWindow {
         bool valid = false;
         uint64_t first = 0, steps = 0;
@@ -387,6 +390,7 @@ struct dp_sim {
         }
         InjectionSlot &slot = injection_slots[next_injection_slot];
         next_injection_slot = (next_injection_slot + 1) % kInjectionSlots;
+        if (window.valid && window.slot == &slot) retire_window();  // the ring came round to it
         if (slot.pending) check(hipEventSynchronize(slot.done), "hipEventSynchronize");
         slot.pending = false;
         const size_t bytes = all.size() * sizeof(InjDesc);
@@ -397,6 +401,15 @@ struct dp_sim {
               "hipMemcpyAsync");
         device_table = slot.table.as<InjDesc>();
         return &slot;
+    }
+
+    // Ends the injection window: its slot stays busy until the stream passes
+    // the launches queued with it.
+    void retire_window() {
+        if (!window.valid) return;
+        window.valid = false;
+        release_injections(window.slot);
+        window.slot = nullptr;
     }
 
     // Marks the slot busy until the stream passes this point (after the
@@ -500,7 +513,7 @@ struct dp_sim {
     void advance(uint64_t steps) {
         if (steps == 0) return;
         if (!window.valid || step < window.first || step + steps > window.first + window.steps) {
-            window.valid = false;
+            retire_window();
             const uint64_t n = std::max<uint64_t>(steps, kInjectionWindowSteps);
             window.table = build_injections(step, n, +1, window.all);
             window.slot = upload_injections(window.all, window.device);
@@ -511,11 +524,6 @@ struct dp_sim {
         const StepInjections *table = window.table.data() + (step - window.first);
         const std::vector<InjDesc> &all = window.all;
         const InjDesc *device_inj = window.device;
-        struct Release {
-            dp_sim *sim;
-            InjectionSlot *slot;
-            ~Release() { sim->release_injections(slot); }
-        } release{this, window.slot};
         if (accumulating) ensure_intensity();
         const int r = effective_rows();
         const bool pairs = kernel == DP_KERNEL_FUSED && (active().flags & DP_TILE_TWO_STEP);
