@@ -244,6 +244,7 @@ struct dp_sim {
         bool pending = false;
     };
     static constexpr int kReadSlots = 2;
+    static_assert(kReadSlots == 2, "wait_reads picks the older of two slots");
     ReadSlot read_slots[kReadSlots];
     int next_read_slot = 0;  // the slot the next readback takes
     hipStream_t read_stream = nullptr;
