@@ -11,7 +11,10 @@ One step = one full leapfrog step of the whole grid (H, injections, E,
 injections, intensity). Prints one JSON line. The default line also carries
 ``batched``, ``served`` (50 req/s) and ``served_200`` (200 req/s) objects: the
 two workloads below, each against the same work run one solve at a time on
-the same GPU (``--no-batched``, ``--no-served`` leave them out), and
+the same GPU (``--no-batched``, ``--no-served`` leave them out),
+``other_workloads`` the bench lines of ``free_space_80`` and
+``graded_scattering_256`` (each run of this script with ``--workload``, in a
+child process; ``--no-other-workloads`` leaves them out), and
 ``cpu_baseline`` times the C oracle on all the run's host threads and on one.
 
 ``--workload batched_uneven_256``: 256 independent uneven solves (sides
@@ -252,6 +255,40 @@ def batch_traffic(run: dict):
         traffic = json.load(handle).get("traffic_bytes_per_launch")
     return traffic, (f"profiles/{os.path.basename(path)}: rocprofv3 FETCH_SIZE x 2 + WRITE_SIZE per launch of this "
                      f"kernel at this shape, from a separate counter run (not measured in this run)")
+
+
+def other_workload(name: str, warmup: int) -> dict:
+    """The bench line of another single-grid workload, from a run of this
+    script with --workload in a child process, summarised."""
+    import subprocess
+
+    cmd = [sys.executable, os.path.abspath(__file__), "--workload", name, "--no-cpu", "--warmup", str(warmup)]
+    log(f"other workload: {name}")
+    try:
+        run = subprocess.run(cmd, capture_output=True, text=True, timeout=600)
+    except subprocess.TimeoutExpired:
+        return {"workload": name, "error": "timed out after 600 s"}
+    lines = [line for line in run.stdout.splitlines() if line.startswith("{")]
+    if run.returncode != 0 or not lines:
+        return {"workload": name, "error": f"exit status {run.returncode}", "stderr_tail": run.stderr[-2000:]}
+    d = json.loads(lines[-1])
+    config = d["config"]
+    roofline = d["roofline"]
+    return {
+        "workload": name,
+        "metric": d["metric"],
+        "value": d["value"],
+        "unit": d["unit"],
+        "steps": d["steps"],
+        "ms_per_step": d["ms_per_step"],
+        "cells": config["cells"],
+        "extent": config["extent"],
+        "readback_every": config.get("readback_every"),
+        "tiling": config.get("tiling"),
+        "roofline": {key: roofline.get(key) for key in ("bound", "achieved", "peak", "unit", "frac", "kernel",
+                                                          "avg_launch_ms")},
+        "note": config.get("note"),
+    }
 
 
 def batched_summary(run: dict) -> dict:
@@ -512,6 +549,8 @@ def main() -> int:
                         help="default workload: skip the batched_uneven_256 comparison")
     parser.add_argument("--no-served", action="store_true",
                         help="default workload: skip the served_stream comparison")
+    parser.add_argument("--no-other-workloads", action="store_true",
+                        help="default workload: skip the free_space_80 and graded_scattering_256 lines")
     parser.add_argument("--cpu-seconds", type=float, default=10.0)
     parser.add_argument("--members", choices=("all", "pec", "absorbing"), default="all",
                         help="batched_uneven_256: all members, or only the PEC cavities or the absorbing boxes")
@@ -744,6 +783,9 @@ def main() -> int:
             # the top of the stated arrival range, where the device, not the
             # arrivals, bounds the served rate
             result["served_200"] = served_summary(run_served(context, 200, 200.0, 32, 0, 0), 200.0, 32)
+        if not args.no_other_workloads:
+            result["other_workloads"] = {name: other_workload(name, args.warmup)
+                                         for name in ("free_space_80", "graded_scattering_256")}
     if not args.no_cpu:
         cpu = cpu_baseline(args.workload, args.cpu_seconds)
         result["cpu_baseline"] = cpu
