@@ -1,11 +1,13 @@
 """Advances queued without waiting, against the oracle bit for bit.
 
-Each `advance` call uploads its injection descriptors into one of a few
-event-guarded slots and returns without waiting for the previous call, so a
-loop of short advances keeps several tables in flight at once. Every step
-must still see its own descriptors: the fields after many back-to-back one-
-and two-step calls (more calls than slots, tables of different sizes, a
-time-dependent source injecting on every step) equal the oracle's exactly.
+An `advance` call outside the sim's injection window uploads the
+descriptors of its next 64 steps (or more) into one of a few event-guarded
+slots and returns without waiting for the previous call; calls inside the
+window reuse them. Every step must still see its own descriptors: the fields
+after many back-to-back one- and two-step calls (windows crossed, tables of
+different sizes, a time-dependent source injecting on every step, a restore
+back into a window, other uploads coming round the slots) equal the oracle's
+exactly.
 """
 from __future__ import annotations
 
@@ -62,6 +64,33 @@ def test_a_long_table_between_short_ones_and_a_reversal(context):
         reference.reverse_by(chunk)
         reference.advance_by(1)
     assert device.step_count() == reference.step_count()
+    assert bits_equal(reference.electric_all, device.read_electric())
+    assert bits_equal(reference.magnetic_all, device.read_magnetic())
+    device.close()
+
+
+def test_the_injection_window_survives_restores_and_other_uploads(context):
+    # A sim keeps the uploaded descriptors of 64 steps and reuses them across
+    # advances. A restore back into the window, autotuner uploads that come
+    # round to the window's slot of the ring, and advances crossing the
+    # window's end must all still apply each step's own descriptors.
+    scene = Scene.photon(Extent(48, 32, 40))
+    reference = oracle.Simulation(scene)
+    device = Simulation(context, scene)
+    tile = (8, 0, TILE_TWO_STEP | TILE_BUFFER)
+    device.set_tile(*tile)
+    device.advance_by(10)
+    e10, h10 = device.read_electric(), device.read_magnetic()
+    device.advance_by(20)
+    device.write_fields(e10, h10, 10)  # back to step 10, inside the window of steps 1-64
+    for _ in range(5):
+        device.autotune(1)  # five more uploads into a ring of four slots
+    device.set_tile(*tile)
+    for chunk in [3, 1, 50, 7, 1]:  # steps 11-72, across the window's end
+        device.advance_by(chunk)
+    reference.advance_by(72)
+    assert device.step_count() == 72
+    assert np.abs(reference.electric_all).max() > 0.0
     assert bits_equal(reference.electric_all, device.read_electric())
     assert bits_equal(reference.magnetic_all, device.read_magnetic())
     device.close()
