@@ -4,7 +4,8 @@ Each draw (grid, boundary, materials, shapes, sources: see random_scenes.py)
 runs through a kernel path and a tiling chosen from the same seed, with the
 |E|² accumulator switched on halfway for every other draw; the fields and the
 f64 intensity must equal the oracle's exactly. A second test advances
-sixteen draws together in one batch, each to its own step count.
+sixteen draws together in one batch, each to its own step count; a third
+runs 300 more draws through the fused forms in chunks of random lengths.
 """
 from __future__ import annotations
 
@@ -101,3 +102,52 @@ def test_random_scenes_batched_to_their_own_step_counts(context):
     batch.close()
     for sim in sims:
         sim.close()
+
+
+def test_many_random_scenes_in_random_chunks(context):
+    # A wider sweep in one test: each draw runs one of the fused forms (one
+    # step per launch, two on one wave, two on stacked waves) at a tiling
+    # from its seed, advanced in chunks of random lengths (crossing injection
+    # windows and odd leftover steps), with the intensity sum switched on
+    # halfway for some. DIAPHANE_SWEEP_CASES widens it (3300 draws have run
+    # clean on MI355X).
+    import os
+
+    count = int(os.environ.get("DIAPHANE_SWEEP_CASES", "300"))
+    failures = []
+    for seed in range(1000, 1000 + count):
+        case = random_case(seed)
+        rng = np.random.default_rng(90_000 + seed)
+        form = int(rng.integers(3))
+        if form == 0:
+            rows, flags = int(rng.choice([4, 6, 8])), int(rng.choice([0, TILE_NONTEMPORAL, TILE_BUFFER]))
+        elif form == 1:
+            rows = int(rng.choice([6, 8, 10]))
+            flags = TILE_TWO_STEP | int(rng.choice([0, TILE_BUFFER, TILE_NONTEMPORAL]))
+        else:
+            rows = int(rng.choice([4, 6]))
+            flags = (TILE_TWO_STEP | TILE_STACK | (int(rng.choice([2, 4])) << 16) | int(rng.choice([0, TILE_BUFFER]))
+                     | int(rng.choice([0, TILE_XCD_REMAP])))
+        depth = int(rng.choice([0, 1, 3, 8, 64]))
+        intensity = bool(rng.integers(2))
+        chunks, left = [], case.steps
+        while left:
+            chunks.append(int(min(left, rng.choice([1, 2, 3, 5, 7, 64, 100]))))
+            left -= chunks[-1]
+        reference = oracle.Simulation(case.scene)
+        device = Simulation(context, case.scene)
+        device.set_tile(rows, depth, flags)
+        for k, chunk in enumerate(chunks):
+            if intensity and k == len(chunks) // 2:
+                reference.set_accumulate_intensity(True)
+                device.set_accumulate_intensity(True)
+            reference.advance_by(chunk)
+            device.advance_by(chunk)
+        same = (bits_equal(reference.electric_all, device.read_electric())
+                and bits_equal(reference.magnetic_all, device.read_magnetic())
+                and (not intensity or np.array_equal(reference.intensity_f64(), device.read_intensity_f64())))
+        device.close()
+        if not same:
+            failures.append(f"{case.describe()} rows {rows} depth {depth} flags {flags:#x} chunks {chunks}")
+    print(f"{count} random scenes in random chunks, {len(failures)} differing", flush=True)
+    assert not failures, "\n".join(failures[:10])
