@@ -5,7 +5,8 @@ runs through a kernel path and a tiling chosen from the same seed, with the
 |E|² accumulator switched on halfway for every other draw; the fields and the
 f64 intensity must equal the oracle's exactly. A second test advances
 sixteen draws together in one batch, each to its own step count; a third
-runs 300 more draws through the fused forms in chunks of random lengths.
+runs 300 more draws through the fused forms in chunks of random lengths, and
+a fourth batches 24 draws at each batch tiling in chunks of random lengths.
 """
 from __future__ import annotations
 
@@ -151,3 +152,48 @@ def test_many_random_scenes_in_random_chunks(context):
             failures.append(f"{case.describe()} rows {rows} depth {depth} flags {flags:#x} chunks {chunks}")
     print(f"{count} random scenes in random chunks, {len(failures)} differing", flush=True)
     assert not failures, "\n".join(failures[:10])
+
+
+BATCH_SHAPES = [
+    (8, 0, TILE_TWO_STEP | TILE_BUFFER | TILE_XCD_REMAP | (4 << 8)),  # the batch default
+    (6, 16, TILE_TWO_STEP),
+    (10, 3, TILE_TWO_STEP | TILE_NONTEMPORAL),
+    (4, 8, TILE_TWO_STEP | TILE_STACK | (4 << 16) | TILE_BUFFER),
+    (6, 0, TILE_TWO_STEP | TILE_STACK | (2 << 16)),
+    (4, 0, TILE_NONTEMPORAL),  # one step per launch, persistent grid
+    (6, 5, 16 | TILE_BUFFER),  # one step per launch, one wave per tile
+    (8, 64, 0),
+]
+
+
+@pytest.mark.parametrize("shape", BATCH_SHAPES, ids=[f"{r}-{d}-{f:#x}" for r, d, f in BATCH_SHAPES])
+def test_random_scenes_batched_on_every_batch_tiling(context, shape):
+    # 24 draws in one batch at each tiling, to their own step counts, advanced
+    # in chunks of random lengths, a third of them summing intensity.
+    b = BATCH_SHAPES.index(shape)
+    cases = [random_case(seed) for seed in range(5000 + 1000 * b, 5024 + 1000 * b)]
+    batch = Batch(context)
+    batch.set_tile(*shape)
+    sims = []
+    for k, case in enumerate(cases):
+        sim = Simulation(context, case.scene)
+        if k % 3 == 0:
+            sim.set_accumulate_intensity(True)
+        batch.add(sim, case.steps)
+        sims.append(sim)
+    rng = np.random.default_rng(b)
+    steps, done = max(case.steps for case in cases), 0
+    while done < steps:
+        chunk = int(min(steps - done, rng.choice([1, 3, 17, 40, 100])))
+        batch.advance_by(chunk)
+        done += chunk
+    batch.wait()
+    for k, (case, sim) in enumerate(zip(cases, sims)):
+        reference = oracle.Simulation(case.scene)
+        if k % 3 == 0:
+            reference.set_accumulate_intensity(True)
+        reference.advance_by(case.steps)
+        assert_matches_oracle(case, reference, sim, k % 3 == 0)
+    batch.close()
+    for sim in sims:
+        sim.close()
