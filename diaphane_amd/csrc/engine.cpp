@@ -227,9 +227,9 @@ int dp_sim_create(dp_engine *engine, const dp_scene *scene, dp_sim **out) {
         // Large grids default to two steps per launch (measured on MI355X:
         // faster from about 384³ up, slower at 256³), where buffer
         // addressing fits; smaller grids keep one step per launch. Two-step
-        // launches give each XCD runs of 8 consecutive workgroups, whose
-        // tiles share halo rows in its L2 (768³, 10 rows: 1% faster plain,
-        // 2% with the intensity sum).
+        // launches give each XCD runs of consecutive workgroups, whose tiles
+        // share halo rows in its L2 (768³, 10 rows and runs of 8: 1% faster
+        // plain, 2% with the intensity sum; stacked tiles take runs of 16).
         if (sim->cells >= dp_sim::kTwoStepDefaultCells &&
             buffer_addressing_fits(sim->cells, g.extent[0], g.extent[1])) {
             sim->tiling[0] = {4, 0, dp_sim::kStackTileFlags | DP_TILE_BUFFER};
@@ -670,7 +670,7 @@ int dp_sim_autotune(dp_sim *sim, uint32_t trials, int *rows, int *depth, int *fl
                 if (!(f & DP_TILE_NONTEMPORAL))
                     for (int w : stack_wave_choices)
                         for (int r : stack_row_choices) {
-                            const int sf = f | dp_sim::kTwoStepTileFlags | DP_TILE_STACK | (w << 16);
+                            const int sf = f | dp_sim::kTwoStepTileFlags | DP_TILE_STACK | (w << 16) | (16 << 8);
                             sweep.push_back({r, d, sf, time_shape(r, d, sf, trials)});
                         }
             }

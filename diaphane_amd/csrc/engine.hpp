@@ -222,10 +222,13 @@ struct dp_sim {
     // XCD-grouped workgroups (DP_TILE_XCD_REMAP, runs of 8).
     static constexpr int kTwoStepTileFlags = DP_TILE_TWO_STEP | DP_TILE_XCD_REMAP;
     // The large-grid default: two steps per launch with four waves of 4
-    // rows stacked along y (16-row tiles, 12 owned), XCD runs of 8. Measured
-    // on MI355X at 768³: 3.18 ms per plain step and 4.38 ms per intensity
-    // step, against 3.34 and 4.75 ms with one 10-row wave per tile.
-    static constexpr int kStackTileFlags = kTwoStepTileFlags | DP_TILE_STACK | (4 << 16);
+    // rows stacked along y (16-row tiles, 12 owned), XCD runs of 16
+    // workgroups. Measured on MI355X at 768³: 3.18 ms per plain step and
+    // 4.38 ms per intensity step, against 3.34 and 4.75 ms with one 10-row
+    // wave per tile; runs of 16 against 8 at 12-plane slabs, 6.15 / 7.90
+    // against 6.15–6.19 / 7.93–7.96 ms per plain / intensity launch (runs of
+    // 2, 4 and 32 were slower).
+    static constexpr int kStackTileFlags = kTwoStepTileFlags | DP_TILE_STACK | (4 << 16) | (16 << 8);
     const Tiling &active() const { return tiling[accumulating ? 1 : 0]; }
     uint64_t launches = 0;
     hipEvent_t events[8] = {};

@@ -389,7 +389,7 @@ int dp_sim_set_kernel(dp_sim *sim, int kernel);
  *                         plane instead of recomputing them.
  * A new sim of at least 768 Ki cells whose offsets fit buffer addressing
  * starts with two steps per launch on stacked tiles (four waves of 4 rows,
- * buffer addressing, XCD runs of 8); smaller ones with one step (4 rows,
+ * buffer addressing, XCD runs of 16); smaller ones with one step (4 rows,
  * non-temporal stores).
  * 0 for rows selects 4; 0 for depth selects a size-aware default (the power
  * of two at or below the number of x-y tiles / 50, within 1..32, or / 25
@@ -403,7 +403,7 @@ int dp_sim_set_tile(dp_sim *sim, int rows, int depth, int flags);
 /* Autotuner for DP_KERNEL_FUSED: times `trials` launches of every rows
  * (4, 6, 8 one step per launch with non-temporal stores; 6, 8, 10 two steps
  * per launch with either store kind and XCD runs of 8; 4 or 6 rows in 2 or 4
- * stacked waves with cached stores and XCD runs of 8) × depth (1 to 64 in powers of two, up
+ * stacked waves with cached stores and XCD runs of 16) × depth (1 to 64 in powers of two, up
  * to twice the grid's z extent) × addressing (pointer, buffer where it fits)
  * candidate on this sim's grid, times the four fastest again with 3·trials
  * launches in two alternating rounds, then three quarters and one and a half
